@@ -1,0 +1,157 @@
+"""Throughput of the RK4 step on one MI355X.
+
+Workload (one "step" = one RK4 step of the whole box): decaying isotropic
+turbulence at 256^3, 3/2-rule dealiasing (384^3 padded), nu = 0.001,
+dt = 0.001, FP32 with FP64 energy/dissipation reductions; the initial field
+is the seeded random-phase solenoidal field with E(k) ~ k^4 exp(-2 (k/4)^2),
+energy 0.5 (synthetic data, generated on the host, resident in HBM before the
+timed region).  The same workload is also timed in FP64.
+
+Prints ONE JSON line with steps/s, the per-kernel HBM roofline of the
+dominant pass (algorithmic bytes per launch / its average HIP-event duration
+on the plan's stream), and a CPU baseline: the NumPy/SciPy oracle timed on
+this host on one RK4 step of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "RK4 steps/s at 256^3 FP32/FP64, % HBM roofline; closure vs DNS speedup @err"
+
+
+def pass_bytes(N, dealias, esz):
+    """Algorithmic bytes per launch of each pass (canonical fused graph).
+
+    F = N0*N1*(N2/2+1) complex elements * esz bytes.  With M = 3N/2 padding
+    (3/2 rule) fields that went through the x (y) inverse are M0/N0 (M1/N1)
+    times larger.  P1 reads u_hat (3F) and writes 5 fields; P2 reads 5 and
+    writes 6; P3 reads 6 and writes 3; P4 reads 3 and writes 3; P5 reads 3 and
+    streams the RK4 state: stage 0 reads u (3F) and writes u0, u2 (6F);
+    stages 1-2 read u1, u0, u2 (9F) and write u0, u2 (6F); stage 3 reads u0,
+    u2 (6F) and writes u (3F)."""
+    F = N[0] * N[1] * (N[2] // 2 + 1) * esz
+    rx = 1.5 if dealias == "3/2-rule" else 1.0
+    rxy = rx * rx
+    p = {
+        "x_inverse": 3 * F + 5 * F * rx,
+        "y_inverse_curl": 5 * F * rx + 6 * F * rxy,
+        "z_cross": 6 * F * rxy + 3 * F * rxy,
+        "y_forward": 3 * F * rxy + 3 * F * rx,
+    }
+    rk = [9 * F, 15 * F, 15 * F, 9 * F]
+    p["x_forward_rk"] = 3 * F * rx + sum(rk) / 4.0
+    step = 4 * (p["x_inverse"] + p["y_inverse_curl"] + p["z_cross"] + p["y_forward"]) + 4 * 3 * F * rx + sum(rk)
+    return p, step, F
+
+
+def gpu_run(N, precision, dealias, steps, warmup, nu=0.001, dt=0.001, seed=1234):
+    import torch
+    from spectraldns_amd._abi import Plan
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    L = (2 * np.pi,) * 3
+    plan = Plan(N, L, precision, dealias)
+    cd = torch.complex64 if precision == "single" else torch.complex128
+    u = torch.from_numpy(random_solenoidal_field(N, L, seed=seed, k0=4.0, energy=0.5)).to(cd).cuda()
+    diag = torch.zeros(2 * max(steps, warmup, 1), dtype=torch.float64, device="cuda")
+    plan.rk4_steps(u, [dt] * warmup, nu, diag=diag)
+    torch.cuda.synchronize()
+    plan.timing_read()
+    plan.timing_enable(True)
+    t0 = time.perf_counter()
+    plan.rk4_steps(u, [dt] * steps, nu, diag=diag)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.timing_enable(False)
+    passes = plan.timing_read()
+    e = diag[: 2 * steps].cpu().numpy().reshape(-1, 2)
+    return elapsed, passes, e
+
+
+def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
+    from oracle.ns import Grid, _WORKERS
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    g = Grid(N, (2 * np.pi,) * 3, precision, dealias)
+    u = random_solenoidal_field(N, seed=1234, k0=4.0, energy=0.5).astype(g.complex)
+    t0 = time.perf_counter()
+    g.rk4_step(u, dt, nu)
+    return time.perf_counter() - t0, _WORKERS
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    N = (args.N,) * 3
+    elapsed, passes, diag = gpu_run(N, "single", args.dealias, args.steps, args.warmup)
+    ms_step = 1000.0 * elapsed / args.steps
+    sps = args.steps / elapsed
+    pb, b_step, F = pass_bytes(N, args.dealias, 8)
+    dom = max((k for k in pb), key=lambda k: passes[k][0])
+    dom_ms, dom_n = passes[dom]
+    avg_s = dom_ms / 1000.0 / max(dom_n, 1)
+    achieved = pb[dom] / avg_s / 1e9
+    per_pass = {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1],
+                    "GBps": round(pb[k] / (v[0] / 1000.0 / max(v[1], 1)) / 1e9, 1) if k in pb and v[1] else None}
+                for k, v in passes.items()}
+    out = {
+        "metric": METRIC,
+        "value": round(sps, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-phase solenoidal field, E(k)~k^4 exp(-2(k/4)^2), energy 0.5)",
+        "config": {"workload": "decaying isotropic turbulence %d^3, %s%s, RK4, nu=0.001, dt=0.001, FP32 "
+                               "state with FP64 energy/dissipation reductions"
+                               % (args.N, args.dealias,
+                                  " (%d^3 padded)" % (3 * args.N // 2) if args.dealias == "3/2-rule" else ""),
+                   "N": args.N, "dealias": args.dealias, "precision": "fp32"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBps": round(b_step * sps / 1e9, 1),
+                          "frac": round(b_step * sps / 1e9 / HBM_PEAK_GBS, 4)},
+        "passes": per_pass,
+        "energy_first_last": [float(diag[0][0]), float(diag[-1][0])],
+    }
+    if not args.no_fp64:
+        s64 = min(args.steps, 20)
+        el64, pass64, _ = gpu_run(N, "double", args.dealias, s64, min(args.warmup, 2))
+        pb64, b64, _ = pass_bytes(N, args.dealias, 16)
+        sps64 = s64 / el64
+        out["fp64"] = {"steps_per_s": round(sps64, 3), "ms_per_step": round(1000 * el64 / s64, 4),
+                       "step_roofline_frac": round(b64 * sps64 / 1e9 / HBM_PEAK_GBS, 4),
+                       "passes": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1]}
+                                  for k, v in pass64.items()}}
+    if not args.no_cpu:
+        t_cpu, cores = cpu_baseline(N, "single", args.dealias)
+        out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": cores, "kind": "port",
+                               "sample": "1 RK4 step of the same %d^3 %s FP32 workload with the NumPy/SciPy "
+                                         "oracle (scipy.fft workers=%d)" % (args.N, args.dealias, cores)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

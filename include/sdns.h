@@ -1,0 +1,119 @@
+/* C-ABI of libsdns_hip.so: the MI355X (gfx950) triply periodic Navier-Stokes
+ * RK4 hot path of spectralDNS, for ctypes / any FFI.
+ *
+ * Conventions
+ *  - Arrays are device pointers (HIP), C order, caller-owned:
+ *      spectral  (ncomp, N0, N1, N2/2+1) complex  (float2 / double2)
+ *      physical  (ncomp, N0, N1, N2)     real     (float / double)
+ *  - Forward transforms are scaled by 1/(N0 N1 N2); backward transforms are
+ *    unscaled (shenfun convention used by spectralDNS/solvers/NS.py).
+ *  - Wavenumbers k_i = (2 pi / L_i) * fftfreq(N_i)*N_i (axes 0, 1) and
+ *    rfftfreq (axis 2), as NS.py:37-41.
+ *  - Every call enqueues work on the plan's HIP stream and returns without
+ *    synchronising.  A plan is not thread-safe; use one plan per thread.
+ *  - Return value 0 on success, a negative SDNS_ERR_* code otherwise; the
+ *    message of the last failure on the calling thread is sdns_last_error().
+ *    No C++ exception crosses this interface and no call falls back to the CPU.
+ */
+#ifndef SDNS_H
+#define SDNS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdns_plan sdns_plan;
+
+enum {
+  SDNS_OK = 0,
+  SDNS_ERR_INVALID = -1,     /* bad argument */
+  SDNS_ERR_UNSUPPORTED = -2, /* size / option without a compiled kernel */
+  SDNS_ERR_HIP = -3,         /* HIP runtime error */
+  SDNS_ERR_NOMEM = -4        /* device allocation failed */
+};
+enum { SDNS_SINGLE = 0, SDNS_DOUBLE = 1 };                                   /* spectralinit.py:25-29 */
+enum { SDNS_DEALIAS_NONE = 0, SDNS_DEALIAS_23 = 1, SDNS_DEALIAS_32 = 2 };    /* config.py:181-183 */
+enum { SDNS_MODE_DIRECT = 0, SDNS_MODE_OUTPUT_MASK = 1 };                   /* 2/3-rule variants */
+
+/* Build the spaces of one box: replaces get_context()'s TensorProductSpace /
+ * VectorSpace / get_dealiased / mask / K / K2 / K_over_K2 set-up
+ * (spectralDNS/solvers/NS.py:12-48).  hip_stream may be NULL (null stream).
+ * dealias_mode selects the 2/3-rule variant: SDNS_MODE_DIRECT truncates
+ * |k_i| >= N_i/3 on the inputs of the dealiased backward transforms
+ * (shenfun dealias_direct), SDNS_MODE_OUTPUT_MASK keeps |k_i| < 2/3(N_i/2+1)
+ * (cpp/spectralDNS.cpp:122-129, spectralDNS3D_short.py:44-46). */
+int sdns_plan_create(sdns_plan** plan, const int32_t N[3], const double L[3], int precision,
+                     int dealias, int mask_nyquist, int dealias_mode, void* hip_stream);
+int sdns_plan_destroy(sdns_plan* plan);
+/* Rebind the plan to another HIP stream (NULL = null stream). */
+int sdns_plan_set_stream(sdns_plan* plan, void* hip_stream);
+/* Device workspace the plan needs; sdns_set_workspace hands it caller memory
+ * (at least sdns_workspace_bytes, 256-byte aligned).  Without it the plan
+ * allocates its own on first use. */
+size_t sdns_workspace_bytes(const sdns_plan* plan);
+int sdns_set_workspace(sdns_plan* plan, void* dptr, size_t bytes);
+
+/* VT.forward / VT.backward of ncomp components (set_velocity, get_velocity,
+ * get_curl, get_pressure, get_divergence: NS.py:86-110). */
+int sdns_forward(sdns_plan* plan, const void* u_phys, void* u_hat, int ncomp);
+int sdns_backward(sdns_plan* plan, const void* u_hat, void* u_phys, int ncomp);
+
+/* Kernel plug-ins of the reference's @optimizer layer
+ * (spectralDNS/optimization/__init__.py:12-55):
+ *  cross1: c = a x b on npoints physical points per component (maths/cross.py:13-25)
+ *  cross2: c_hat = 1j*(K x a_hat) (maths/cross.py:27-32)
+ *  add_pressure_diffusion: NS.py:203-217 in place on rhs; P_hat may be NULL
+ *  divergence: div_hat = 1j*(K . u_hat) (NS.py:106-110)
+ *  mul_minus_i: out = -1j*a on n complex points (NS.py:96-99)
+ *  mask_nyquist: zero the Nyquist planes of a (3, N0, N1, Nh) array (NS.py:253-254) */
+int sdns_cross1(sdns_plan* plan, void* c, const void* a, const void* b, size_t npoints);
+int sdns_cross2(sdns_plan* plan, void* c_hat, const void* a_hat);
+int sdns_add_pressure_diffusion(sdns_plan* plan, void* rhs, const void* u_hat, double nu, void* p_hat);
+int sdns_divergence(sdns_plan* plan, void* div_hat, const void* u_hat);
+int sdns_mul_minus_i(sdns_plan* plan, void* out, const void* a, size_t n);
+int sdns_mask_nyquist(sdns_plan* plan, void* a_hat);
+
+/* Conv of getConvection('Vortex') (NS.py:191-198): rhs = VTp.forward(u x curl u)
+ * on the dealiased space, without mask, pressure or diffusion. */
+int sdns_convection(sdns_plan* plan, const void* u_hat, void* rhs);
+
+/* ComputeRHS (NS.py:219-261) with the Vortex convection (NS.py:191-198):
+ * dealiased u x curl(u), Nyquist mask, pressure and diffusion, + source
+ * (source and p_hat may be NULL). */
+int sdns_compute_rhs(sdns_plan* plan, const void* u_hat, void* rhs, double nu, const void* source,
+                     void* p_hat);
+
+/* nsteps classical RK4 steps in place on u_hat (integrators.py:149-159,
+ * coefficients in the working precision :185-186), step s using dt_seq[s]
+ * (host array, the dt sequence of the solve() loop).  diag, when not NULL,
+ * is a device array of 2*nsteps doubles receiving after every step the
+ * kinetic energy 0.5*sum w|u_hat|^2 and the dissipation nu*sum w|K x u_hat|^2
+ * (FP64 reductions; w = 1 on kz = 0, N2/2 and 2 elsewhere).
+ * u_lo: optional low-order words of u_hat for compensated single-precision
+ * accumulation (NULL = plain).  p_hat, when not NULL, receives the pressure of
+ * the last RHS evaluation, as the reference leaves P_hat after a step. */
+int sdns_rk4_steps(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_seq, int nsteps, double nu,
+                   double* diag, void* p_hat);
+
+/* Kinetic energy and dissipation of u_hat in FP64 into the device array out[2]. */
+int sdns_energy(sdns_plan* plan, const void* u_hat, double nu, double* out);
+
+/* Per-pass GPU time from HIP events recorded on the plan's stream around
+ * every pass launch (benchmark support).  sdns_timing_enable(plan, 1) starts
+ * recording; sdns_timing_read waits for the recorded events, writes the summed
+ * milliseconds and launch counts per pass id (0: x-inverse, 1: y-inverse +
+ * curl, 2: z pass + cross product, 3: y-forward, 4: x-forward + projection +
+ * RK stage, 5: other launches) and clears the record. */
+int sdns_timing_enable(sdns_plan* plan, int enable);
+int sdns_timing_read(sdns_plan* plan, double ms[6], int count[6]);
+
+/* Human-readable message of the last failing call on this thread. */
+const char* sdns_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDNS_H */

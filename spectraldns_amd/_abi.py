@@ -1,0 +1,266 @@
+"""ctypes binding of libsdns_hip.so (C-ABI declared in include/sdns.h).
+
+There is no CPU fallback: if the library is missing or fails to load, every
+entry point raises.  Tensors are torch tensors on the GPU; only their device
+pointers cross the ABI.  Shapes, dtypes and contiguity are checked here on the
+host before any kernel is launched.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdns_hip.so")
+
+SDNS_SINGLE, SDNS_DOUBLE = 0, 1
+DEALIAS = {"None": 0, None: 0, "2/3-rule": 1, "3/2-rule": 2}
+
+# every exported symbol of include/sdns.h with (restype, argtypes)
+_P = ctypes.c_void_p
+_SIGS = {
+    "sdns_plan_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _P]),
+    "sdns_plan_destroy": (ctypes.c_int, [_P]),
+    "sdns_plan_set_stream": (ctypes.c_int, [_P, _P]),
+    "sdns_workspace_bytes": (ctypes.c_size_t, [_P]),
+    "sdns_set_workspace": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "sdns_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
+    "sdns_backward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
+    "sdns_cross1": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_size_t]),
+    "sdns_cross2": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_add_pressure_diffusion": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
+    "sdns_divergence": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_mul_minus_i": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "sdns_mask_nyquist": (ctypes.c_int, [_P, _P]),
+    "sdns_convection": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_compute_rhs": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
+    "sdns_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                      ctypes.c_double, _P, _P]),
+    "sdns_energy": (ctypes.c_int, [_P, _P, ctypes.c_double, _P]),
+    "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "sdns_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class SdnsError(RuntimeError):
+    """A libsdns_hip call returned an error code."""
+
+
+def load():
+    """Load libsdns_hip.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SdnsError("libsdns_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SdnsError("libsdns_hip error %d: %s" % (rc, _lib.sdns_last_error().decode()))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Plan:
+    """One box: sizes, dealiasing, precision, twiddles, workspace (NS.py:12-48)."""
+
+    def __init__(self, N, L, precision="double", dealias="2/3-rule", mask_nyquist=True, dealias_mode=0):
+        import torch
+        lib = load()
+        if not torch.cuda.is_available():
+            raise SdnsError("no GPU visible: the HIP path has no CPU fallback")
+        self.N = tuple(int(n) for n in N)
+        self.L = tuple(float(x) for x in L)
+        self.precision = precision
+        self.dealias = dealias
+        self.dealias_mode = int(dealias_mode)
+        self.mask_nyquist = bool(mask_nyquist)
+        self.real_dtype = torch.float32 if precision == "single" else torch.float64
+        self.complex_dtype = torch.complex64 if precision == "single" else torch.complex128
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        h = _P()
+        Na = (ctypes.c_int32 * 3)(*self.N)
+        La = (ctypes.c_double * 3)(*self.L)
+        stream = torch.cuda.current_stream().cuda_stream
+        _check(lib.sdns_plan_create(ctypes.byref(h), Na, La,
+                                    SDNS_SINGLE if precision == "single" else SDNS_DOUBLE,
+                                    DEALIAS[dealias], int(self.mask_nyquist), self.dealias_mode,
+                                    _P(stream)))
+        self._h = h
+        self._lib = lib
+        nbytes = lib.sdns_workspace_bytes(h)
+        # workspace from torch's caching allocator (256-byte aligned)
+        self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
+        base = self._ws.data_ptr()
+        off = (-base) % 256
+        _check(lib.sdns_set_workspace(h, _P(base + off), nbytes))
+        self.workspace_bytes = nbytes
+        self.spectral_shape = (self.N[0], self.N[1], self.N[2] // 2 + 1)
+        self.physical_shape = self.N
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.sdns_plan_destroy(h)
+            self._h = None
+
+    # ---------------------------------------------------------------- checks
+    def _sync_stream(self):
+        import torch
+        _check(self._lib.sdns_plan_set_stream(self._h, _P(torch.cuda.current_stream().cuda_stream)))
+
+    def _spec(self, t, ncomp=None, name="array"):
+        if t.device.type != "cuda" or t.dtype != self.complex_dtype or not t.is_contiguous():
+            raise SdnsError("%s must be a contiguous %s tensor on the GPU" % (name, self.complex_dtype))
+        if tuple(t.shape[-3:]) != self.spectral_shape:
+            raise SdnsError("%s has shape %s, expected (..., %d, %d, %d)" % ((name, tuple(t.shape)) + self.spectral_shape))
+        nc = 1 if t.dim() == 3 else t.shape[0]
+        if t.dim() not in (3, 4) or (ncomp is not None and nc != ncomp):
+            raise SdnsError("%s must have %s components" % (name, ncomp))
+        return nc
+
+    def _phys(self, t, name="array"):
+        if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous():
+            raise SdnsError("%s must be a contiguous %s tensor on the GPU" % (name, self.real_dtype))
+        if tuple(t.shape[-3:]) != self.physical_shape or t.dim() not in (3, 4):
+            raise SdnsError("%s has shape %s, expected (..., %d, %d, %d)" % ((name, tuple(t.shape)) + self.physical_shape))
+        return 1 if t.dim() == 3 else t.shape[0]
+
+    # ---------------------------------------------------------------- entry points
+    def forward(self, u, u_hat):
+        nc = self._phys(u, "u")
+        if self._spec(u_hat, nc, "u_hat") != nc:
+            raise SdnsError("component mismatch")
+        self._sync_stream()
+        _check(self._lib.sdns_forward(self._h, _ptr(u), _ptr(u_hat), nc))
+        return u_hat
+
+    def backward(self, u_hat, u):
+        nc = self._spec(u_hat, None, "u_hat")
+        if self._phys(u, "u") != nc:
+            raise SdnsError("component mismatch")
+        self._sync_stream()
+        _check(self._lib.sdns_backward(self._h, _ptr(u_hat), _ptr(u), nc))
+        return u
+
+    def cross1(self, c, a, b):
+        for t, nm in ((c, "c"), (a, "a"), (b, "b")):
+            if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous() or t.shape[0] != 3:
+                raise SdnsError("cross1: %s must be a contiguous (3, ...) %s GPU tensor" % (nm, self.real_dtype))
+        if not (c.shape == a.shape == b.shape):
+            raise SdnsError("cross1: shape mismatch")
+        self._sync_stream()
+        _check(self._lib.sdns_cross1(self._h, _ptr(c), _ptr(a), _ptr(b), a[0].numel()))
+        return c
+
+    def cross2(self, c_hat, a_hat):
+        self._spec(c_hat, 3, "c")
+        self._spec(a_hat, 3, "a")
+        self._sync_stream()
+        _check(self._lib.sdns_cross2(self._h, _ptr(c_hat), _ptr(a_hat)))
+        return c_hat
+
+    def add_pressure_diffusion(self, rhs, u_hat, nu, p_hat=None):
+        self._spec(rhs, 3, "rhs")
+        self._spec(u_hat, 3, "u_hat")
+        if p_hat is not None:
+            self._spec(p_hat, 1, "P_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_add_pressure_diffusion(self._h, _ptr(rhs), _ptr(u_hat), float(nu),
+                                                      _ptr(p_hat) if p_hat is not None else None))
+        return rhs
+
+    def divergence(self, div_hat, u_hat):
+        self._spec(div_hat, 1, "div_hat")
+        self._spec(u_hat, 3, "u_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_divergence(self._h, _ptr(div_hat), _ptr(u_hat)))
+        return div_hat
+
+    def mul_minus_i(self, out, a):
+        if out.shape != a.shape or out.dtype != self.complex_dtype or a.dtype != self.complex_dtype \
+                or not (out.is_contiguous() and a.is_contiguous()):
+            raise SdnsError("mul_minus_i: matching contiguous complex tensors required")
+        self._sync_stream()
+        _check(self._lib.sdns_mul_minus_i(self._h, _ptr(out), _ptr(a), a.numel()))
+        return out
+
+    def mask_nyquist(self, a_hat):
+        self._spec(a_hat, 3, "a_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_mask_nyquist(self._h, _ptr(a_hat)))
+        return a_hat
+
+    def convection(self, u_hat, rhs):
+        self._spec(u_hat, 3, "u_hat")
+        self._spec(rhs, 3, "rhs")
+        self._sync_stream()
+        _check(self._lib.sdns_convection(self._h, _ptr(u_hat), _ptr(rhs)))
+        return rhs
+
+    def compute_rhs(self, u_hat, rhs, nu, source=None, p_hat=None):
+        self._spec(u_hat, 3, "u_hat")
+        self._spec(rhs, 3, "rhs")
+        if source is not None:
+            self._spec(source, 3, "Source")
+        if p_hat is not None:
+            self._spec(p_hat, 1, "P_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_compute_rhs(self._h, _ptr(u_hat), _ptr(rhs), float(nu),
+                                          _ptr(source) if source is not None else None,
+                                          _ptr(p_hat) if p_hat is not None else None))
+        return rhs
+
+    def rk4_steps(self, u_hat, dts, nu, diag=None, u_lo=None, p_hat=None):
+        import torch
+        self._spec(u_hat, 3, "u_hat")
+        n = len(dts)
+        if diag is not None:
+            if diag.dtype != torch.float64 or diag.device.type != "cuda" or diag.numel() < 2 * n \
+                    or not diag.is_contiguous():
+                raise SdnsError("diag must be a contiguous float64 GPU tensor of >= 2*nsteps entries")
+        if u_lo is not None:
+            self._spec(u_lo, 3, "u_lo")
+        if p_hat is not None:
+            self._spec(p_hat, 1, "P_hat")
+        arr = (ctypes.c_double * max(n, 1))(*[float(d) for d in dts])
+        self._sync_stream()
+        _check(self._lib.sdns_rk4_steps(self._h, _ptr(u_hat), _ptr(u_lo) if u_lo is not None else None,
+                                        arr, n, float(nu), _ptr(diag) if diag is not None else None,
+                                        _ptr(p_hat) if p_hat is not None else None))
+        return u_hat
+
+    def energy(self, u_hat, nu, out):
+        import torch
+        self._spec(u_hat, 3, "u_hat")
+        if out.dtype != torch.float64 or out.numel() < 2 or out.device.type != "cuda":
+            raise SdnsError("out must be a float64 GPU tensor of 2 entries")
+        self._sync_stream()
+        _check(self._lib.sdns_energy(self._h, _ptr(u_hat), float(nu), _ptr(out)))
+        return out
+
+    # ---------------------------------------------------------------- timing
+    PASS_NAMES = ("x_inverse", "y_inverse_curl", "z_cross", "y_forward", "x_forward_rk", "other")
+
+    def timing_enable(self, on=True):
+        _check(self._lib.sdns_timing_enable(self._h, int(bool(on))))
+
+    def timing_read(self):
+        """{pass name: (milliseconds, launches)} since the last read."""
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_int * 6)()
+        _check(self._lib.sdns_timing_read(self._h, ms, cnt))
+        return {n: (ms[i], cnt[i]) for i, n in enumerate(self.PASS_NAMES)}

@@ -1,0 +1,514 @@
+// Pass kernels of the fused pseudo-spectral RK4 step and their launchers.
+//
+// One RHS evaluation (spectralDNS/solvers/NS.py:219-261, Vortex form
+// NS.py:191-198) is five launches that each stream HBM once:
+//   P1 x-inverse of u_hat (+ kx*w_hat, kx*v_hat for the curl)  [strided, x]
+//   P2 y-inverse and curl assembly i(K x u_hat)                [strided, y]
+//   P3 z C2R of 6 fields, u x curl(u), z R2C of 3 fields        [contiguous, z]
+//   P4 y-forward                                                 [strided, y]
+//   P5 x-forward, Nyquist mask, pressure projection, diffusion,
+//      RK4 stage update (integrators.py:149-159) and, on the last
+//      stage, the FP64 energy / enstrophy partial sums           [strided, x]
+// Physical space never touches HBM.  Dealiasing is folded into the row maps:
+// truncated modes are neither read nor written, padded bins are zeros.
+#pragma once
+#include "sdns_device.hpp"
+#include "sdns_types.hpp"
+
+namespace sdns {
+
+// ------------------------------------------------------------------ tiling
+// Columns per block: about 256 threads, LDS tile M*T <= 64 KB, and T*P a
+// multiple of 64 so that every block is made of whole waves (the P5 energy
+// reduction shuffles across full 64-lane waves).
+constexpr int strided_T(int M, int P, int esz) {
+  int T0 = 256 / P;
+  if (T0 < 1) T0 = 1;
+  if (T0 > 64) T0 = 64;
+  for (int T = T0; T >= 1; --T)
+    if ((T * P) % 64 == 0 && M * T * esz <= 65536) return T;
+  for (int T = 64; T >= 1; --T)
+    if ((T * P) % 64 == 0 && M * T * esz <= 65536) return T;
+  return 0;
+}
+template <typename Rt, int M> struct SCfg {
+  static constexpr int E = strided_E(M);
+  static constexpr int P = M / E;
+  static constexpr int T = strided_T(M, P, (int)sizeof(cx<Rt>));
+  static constexpr int THREADS = T * P;
+};
+
+constexpr int ZPAD = 1;
+constexpr int z_lpb(int n, int P, int esz) {
+  int L = 256 / P;
+  if (L < 1) L = 1;
+  while (L > 1 && L * (n + ZPAD) * esz > 32768) L /= 2;
+  return L;
+}
+template <typename Rt, int n> struct ZCfg {
+  static constexpr int E = zline_E(n);
+  static constexpr int P = n / E;
+  static constexpr int LPB = z_lpb(n, P, (int)sizeof(cx<Rt>));
+  static constexpr int THREADS = LPB * P;
+};
+
+__device__ __forceinline__ int bin_row(int p, int M, RowMap m) {
+  if (p <= m.pos) return p;
+  if (p >= M - m.neg) return m.base + (p - M);
+  return -1;
+}
+template <typename C> __device__ __forceinline__ C czero() { return mk<C>(0, 0); }
+
+// ------------------------------------------------------------------ P1
+template <typename Rt, int M>
+__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p1(P1Args<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = SCfg<Rt, M>;
+  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using F = LineFFT<Rt, M, E, +1>;
+  __shared__ C lds[M * T];
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  const int q = blockIdx.x * T + c;
+  const bool valid = q < a.Q;
+  const int qq = valid ? q : 0;
+  const int iyc = qq / a.nkz, kzi = qq - iyc * a.nkz;
+  const int nky = a.y_neg + a.y_pos + 1;
+  const int iy = iyc <= a.y_pos ? iyc : a.N1 - (nky - iyc);
+  const long plane = (long)a.N1 * a.Nh, fs = (long)a.N0 * plane;
+  const C* __restrict__ U = (const C*)a.u + (long)iy * a.Nh + kzi;
+  const C* __restrict__ tw = (const C*)a.tw;
+  long src[E];
+  Rt kxe[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int r = bin_row(t + P * e, M, a.x);
+    src[e] = (valid && r >= 0) ? r * plane : -1;
+    kxe[e] = r >= 0 ? a.kx[r] : Rt(0);
+  }
+  C* __restrict__ out = (C*)a.w1 + q;
+  const long orow = a.Q, ofield = (long)M * a.Q;
+  C* line = lds + c;
+  auto store = [&](int f, C(&v)[E]) {
+    if (valid) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * orow] = v[e];
+    }
+  };
+  C v[E], x[E];
+  // component 2: X(kx w) -> field 3, X(w) -> field 2
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[2 * fs + src[e]] : czero<C>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = cscale(v[e], kxe[e]);
+  F::run(x, t, line, T, tw);
+  store(3, x);
+  F::run(v, t, line, T, tw);
+  store(2, v);
+  // component 1: X(kx v) -> field 4, X(v) -> field 1
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[fs + src[e]] : czero<C>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = cscale(v[e], kxe[e]);
+  F::run(x, t, line, T, tw);
+  store(4, x);
+  F::run(v, t, line, T, tw);
+  store(1, v);
+  // component 0
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[src[e]] : czero<C>();
+  F::run(v, t, line, T, tw);
+  store(0, v);
+}
+
+// ------------------------------------------------------------------ P2
+// curl(u)^ = i (K x u_hat) (maths/cross.py:27-32) assembled after the
+// x-inverse: wx = i(ky W - kz V), wy = i(kz U - X(kx W)), wz = i(X(kx V) - ky U).
+template <typename Rt, int M>
+__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p2(P2Args<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = SCfg<Rt, M>;
+  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using F = LineFFT<Rt, M, E, +1>;
+  __shared__ C lds[M * T];
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  const int q = blockIdx.x * T + c;
+  const bool valid = q < a.Q;
+  const int qq = valid ? q : 0;
+  const int x = qq / a.nkz, kzi = qq - x * a.nkz;
+  const int nky = a.y.base;
+  const long ifield = (long)a.M0 * nky * a.nkz, ofield = (long)a.M0 * M * a.nkz;
+  const C* __restrict__ in = (const C*)a.w1 + (long)x * nky * a.nkz + kzi;
+  C* __restrict__ out = (C*)a.w2 + (long)x * M * a.nkz + kzi;
+  const C* __restrict__ tw = (const C*)a.tw;
+  const long rs = a.nkz;
+  long src[E];
+  Rt kye[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    const int r = bin_row(p, M, a.y);
+    src[e] = (valid && r >= 0) ? r * rs : -1;
+    const int k = p <= a.y.pos ? p : p - M;
+    kye[e] = r >= 0 ? a.ky[k >= 0 ? k : a.N1 + k] : Rt(0);
+  }
+  const Rt kzv = a.kz[kzi];
+  C* line = lds + c;
+  auto store = [&](int f, C(&v)[E]) {
+    if (valid) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * rs] = v[e];
+    }
+  };
+  auto load = [&](int f, C(&v)[E]) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? in[f * ifield + src[e]] : czero<C>();
+  };
+  C A[E], B[E], om[E];
+  load(1, A);  // V
+  load(2, B);  // W
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    om[e] = cmuli<1>(mk<C>(kye[e] * B[e].x - kzv * A[e].x, kye[e] * B[e].y - kzv * A[e].y));
+  F::run(om, t, line, T, tw);
+  store(3, om);
+  F::run(A, t, line, T, tw);
+  store(1, A);
+  F::run(B, t, line, T, tw);
+  store(2, B);
+  load(0, A);  // U
+  load(4, B);  // X(kx V)
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    om[e] = cmuli<1>(mk<C>(B[e].x - kye[e] * A[e].x, B[e].y - kye[e] * A[e].y));
+  F::run(om, t, line, T, tw);
+  store(5, om);
+  load(3, B);  // X(kx W)
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    om[e] = cmuli<1>(mk<C>(kzv * A[e].x - B[e].x, kzv * A[e].y - B[e].y));
+  F::run(om, t, line, T, tw);
+  store(4, om);
+  F::run(A, t, line, T, tw);
+  store(0, A);
+}
+
+// ------------------------------------------------------------------ z-line real transforms
+// Length-M2 real line <-> half-length complex FFT (n = M2/2):
+//  C2R: Z[p] = (X[p] + conj X[n-p]) + i e^{+2 pi i p/M2} (X[p] - conj X[n-p]),
+//       z = IFFT_n(Z) = x[2m] + i x[2m+1]   (imaginary parts of X[0], X[n] ignored,
+//       as numpy/FFTW c2r do)
+//  R2C: Z = FFT_n(x[2m] + i x[2m+1]),
+//       X[k] = (Z[k] + conj Z[n-k])/2 - (i/2) e^{-2 pi i k/M2} (Z[k] - conj Z[n-k]).
+template <typename Rt, int n, int E>
+__device__ __forceinline__ void c2r_pre(cx<Rt> (&z)[E], const cx<Rt>* __restrict__ in, int nk, bool valid,
+                                        const cx<Rt>* __restrict__ rtw, int t) {
+  using C = cx<Rt>;
+  constexpr int P = n / E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    const int pn = n - p;
+    C xa = (valid && p < nk) ? in[p] : czero<C>();
+    C xb = (valid && pn < nk) ? in[pn] : czero<C>();
+    if (p == 0) { xa.y = 0; xb.y = 0; }
+    const C ev = mk<C>(xa.x + xb.x, xa.y - xb.y);
+    const C d = mk<C>(xa.x - xb.x, xa.y + xb.y);
+    const C od = cmulc(d, rtw[p]);
+    z[e] = mk<C>(ev.x - od.y, ev.y + od.x);
+  }
+}
+
+// Post-process of the forward half-length FFT held in z (natural positions
+// t + P e); `my` is this line's LDS area.  Writes kept bins k < nk (and the
+// Nyquist bin n when nk > n).
+template <typename Rt, int n, int E>
+__device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __restrict__ out, int nk, bool valid,
+                                         const cx<Rt>* __restrict__ rtw, int t, Rt scale) {
+  using C = cx<Rt>;
+  constexpr int P = n / E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) my[t + P * e] = z[e];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + P * e;
+    const C A = z[e];
+    const C Bz = my[(n - k) % n];
+    const C s = mk<C>(A.x + Bz.x, A.y - Bz.y);   // A + conj(B)
+    const C d = mk<C>(A.x - Bz.x, A.y + Bz.y);   // A - conj(B)
+    const C wd = cmul(rtw[k], d);
+    const C X = mk<C>(Rt(0.5) * (s.x + wd.y), Rt(0.5) * (s.y - wd.x));
+    if (valid && k < nk) out[k] = cscale(X, scale);
+    if (valid && k == 0 && nk > n) out[n] = mk<C>((A.x - A.y) * scale, Rt(0));
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ P3
+template <typename Rt, int n>
+__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_p3(P3Args<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = ZCfg<Rt, n>;
+  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Fi = LineFFT<Rt, n, E, +1>;
+  using Ff = LineFFT<Rt, n, E, -1>;
+  __shared__ C lds[LPB * (n + ZPAD)];
+  const int t = threadIdx.x % P, l = threadIdx.x / P;
+  const int line = blockIdx.x * LPB + l;
+  const bool valid = line < a.L;
+  const long li = valid ? line : 0;
+  C* my = lds + l * (n + ZPAD);
+  const C* __restrict__ tw = (const C*)a.tw;
+  const C* __restrict__ rtw = (const C*)a.rtw;
+  const C* __restrict__ in = (const C*)a.w2 + li * a.nkz;
+  const long ifield = (long)a.L * a.nkz;
+  C ph[6][E];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    c2r_pre<Rt, n, E>(ph[f], in + f * ifield, a.nkz, valid, rtw, t);
+    Fi::run(ph[f], t, my, 1, tw);
+  }
+  // u x w on the packed points (real part: x[2m], imaginary part: x[2m+1])
+  C cr[3][E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const C u0 = ph[0][e], u1 = ph[1][e], u2 = ph[2][e];
+    const C w0 = ph[3][e], w1 = ph[4][e], w2 = ph[5][e];
+    cr[0][e] = mk<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+    cr[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+    cr[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+  }
+  C* __restrict__ out = (C*)a.w3 + li * a.nkzo;
+  const long ofield = (long)a.L * a.nkzo;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Ff::run(cr[i], t, my, 1, tw);
+    r2c_post<Rt, n, E>(cr[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+  }
+}
+
+// ------------------------------------------------------------------ plain z transforms
+template <typename Rt, int n>
+__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = ZCfg<Rt, n>;
+  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Fi = LineFFT<Rt, n, E, +1>;
+  __shared__ C lds[LPB * (n + ZPAD)];
+  const int t = threadIdx.x % P, l = threadIdx.x / P;
+  const int line = blockIdx.x * LPB + l;
+  const bool valid = line < a.L;
+  const long li = valid ? line : 0;
+  C* my = lds + l * (n + ZPAD);
+  const C* in = (const C*)a.in + blockIdx.y * a.in_field + li * a.in_line;
+  C z[E];
+  c2r_pre<Rt, n, E>(z, in, a.nk, valid, (const C*)a.rtw, t);
+  Fi::run(z, t, my, 1, (const C*)a.tw);
+  if (valid) {
+    C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;  // real line viewed as n complex pairs
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[t + P * e] = cscale(z[e], a.scale);
+  }
+}
+
+template <typename Rt, int n>
+__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_r2c(ZArgs<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = ZCfg<Rt, n>;
+  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Ff = LineFFT<Rt, n, E, -1>;
+  __shared__ C lds[LPB * (n + ZPAD)];
+  const int t = threadIdx.x % P, l = threadIdx.x / P;
+  const int line = blockIdx.x * LPB + l;
+  const bool valid = line < a.L;
+  const long li = valid ? line : 0;
+  C* my = lds + l * (n + ZPAD);
+  const C* in = (const C*)a.in + blockIdx.y * a.in_field + li * a.in_line;
+  C z[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) z[e] = valid ? in[t + P * e] : czero<C>();
+  Ff::run(z, t, my, 1, (const C*)a.tw);
+  C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;
+  r2c_post<Rt, n, E>(z, my, out, a.nk, valid, (const C*)a.rtw, t, a.scale);
+}
+
+// ------------------------------------------------------------------ generic strided
+template <typename Rt, int M, int DIR>
+__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_strided(StridedArgs<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = SCfg<Rt, M>;
+  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using F = LineFFT<Rt, M, E, DIR>;
+  __shared__ C lds[M * T];
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  const int q = blockIdx.x * T + c;
+  const bool valid = q < a.Q;
+  const int qq = valid ? q : 0;
+  const int hi = qq / a.cdiv, lo = qq - hi * a.cdiv;
+  const C* __restrict__ in = (const C*)a.in + blockIdx.y * a.in_field + hi * a.in_hi + lo;
+  C* __restrict__ out = (C*)a.out + blockIdx.y * a.out_field + hi * a.out_hi + lo;
+  C v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int r = bin_row(t + P * e, M, a.rin);
+    v[e] = (valid && r >= 0) ? in[(long)r * a.in_row] : czero<C>();
+  }
+  F::run(v, t, lds + c, T, (const C*)a.tw);
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int r = bin_row(t + P * e, M, a.rout);
+      if (r >= 0) out[(long)r * a.out_row] = cscale(v[e], a.scale);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ P5
+template <int MODE> struct P5Tag {};
+
+template <typename Rt, int M, int MODE>
+__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p5(P5Args<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = SCfg<Rt, M>;
+  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T, TH = Cfg::THREADS;
+  using F = LineFFT<Rt, M, E, -1>;
+  __shared__ C lds[M * T];
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  const int q = blockIdx.x * T + c;
+  const bool valid = q < a.Q;
+  const int qq = valid ? q : 0;
+  const int iy = qq / a.Nh, kz = qq - iy * a.Nh;
+  const int ky = iy < a.N1 / 2 ? iy : iy - a.N1;
+  const int nkyo = a.y_neg + a.y_pos + 1;
+  const bool colconv = valid && ky >= -a.y_neg && ky <= a.y_pos && kz < a.nkzo;
+  const int kyc = ky >= 0 ? ky : nkyo + ky;
+  const long wrow = (long)nkyo * a.nkzo, wfield = (long)M * wrow;
+  const C* __restrict__ w4 = (const C*)a.w4 + (colconv ? (long)kyc * a.nkzo + kz : 0);
+  const C* __restrict__ tw = (const C*)a.tw;
+  C X0[E], X1[E], X2[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const long o = (long)(t + P * e) * wrow;
+    X0[e] = colconv ? w4[o] : czero<C>();
+    X1[e] = colconv ? w4[wfield + o] : czero<C>();
+    X2[e] = colconv ? w4[2 * wfield + o] : czero<C>();
+  }
+  C* line = lds + c;
+  F::run(X0, t, line, T, tw);
+  F::run(X1, t, line, T, tw);
+  F::run(X2, t, line, T, tw);
+
+  const long plane = (long)a.N1 * a.Nh, fs = (long)a.N0 * plane;
+  const Rt K1 = a.ky[iy], K2c = a.kz[kz];
+  const bool nyq_col = a.mask_nyquist && (iy == a.N1 / 2 || kz == a.N2 / 2);
+  double esum = 0.0, osum = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    int ix;
+    bool kept;
+    if (M == a.N0) {
+      ix = p;
+      const int k = p < a.N0 / 2 ? p : p - a.N0;
+      kept = k >= -a.x.neg && k <= a.x.pos;
+    } else {
+      ix = bin_row(p, M, a.x);
+      kept = true;
+    }
+    if (!valid || ix < 0) continue;
+    const long off = (long)ix * plane + q;
+    if constexpr (MODE == P5_RHS) {
+      if (a.conv_only) {
+        const bool cv = colconv && kept;
+        C* rhs = (C*)a.rhs;
+        rhs[off] = cv ? cscale(X0[e], a.scale) : czero<C>();
+        rhs[fs + off] = cv ? cscale(X1[e], a.scale) : czero<C>();
+        rhs[2 * fs + off] = cv ? cscale(X2[e], a.scale) : czero<C>();
+        continue;
+      }
+    }
+    const bool conv = colconv && kept && !(nyq_col || (a.mask_nyquist && ix == a.N0 / 2));
+    C r0 = conv ? cscale(X0[e], a.scale) : czero<C>();
+    C r1 = conv ? cscale(X1[e], a.scale) : czero<C>();
+    C r2 = conv ? cscale(X2[e], a.scale) : czero<C>();
+    const Rt K0 = a.kx[ix];
+    Rt k2 = K0 * K0;
+    k2 += K1 * K1;
+    k2 += K2c * K2c;
+    const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+    const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2c / den;
+    // P_hat = sum_i rhs_i K_i/K^2 ; rhs_i -= P_hat K_i ; rhs -= (nu K^2) u_hat  (NS.py:208-215)
+    C ph = mk<C>(r0.x * o0 + r1.x * o1, r0.y * o0 + r1.y * o1);
+    ph = mk<C>(ph.x + r2.x * o2, ph.y + r2.y * o2);
+    r0 = mk<C>(r0.x - ph.x * K0, r0.y - ph.y * K0);
+    r1 = mk<C>(r1.x - ph.x * K1, r1.y - ph.y * K1);
+    r2 = mk<C>(r2.x - ph.x * K2c, r2.y - ph.y * K2c);
+    const Rt z = a.nu * k2;
+    const C* ust = (const C*)(MODE == P5_RK0 || MODE == P5_RHS ? a.A : a.B);
+    const C s0 = ust[off], s1 = ust[fs + off], s2 = ust[2 * fs + off];
+    r0 = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
+    r1 = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
+    r2 = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
+    if constexpr (MODE == P5_RHS) {
+      C* rhs = (C*)a.rhs;
+      if (a.source) {
+        const C* sr = (const C*)a.source;
+        r0 = cadd(r0, sr[off]); r1 = cadd(r1, sr[fs + off]); r2 = cadd(r2, sr[2 * fs + off]);
+      }
+      rhs[off] = r0; rhs[fs + off] = r1; rhs[2 * fs + off] = r2;
+      if (a.phat) ((C*)a.phat)[off] = ph;
+    } else {
+      C* A = (C*)a.A;
+      C* B = (C*)a.B;
+      C* Cc = (C*)a.Cacc;
+      const Rt bdt = a.bdt, adt = a.adt;
+      auto upd = [](C base, C r, Rt s) { return mk<C>(base.x + s * r.x, base.y + s * r.y); };
+      if constexpr (MODE == P5_RK0) {
+        // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
+        B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
+        Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
+      } else if constexpr (MODE == P5_RK12) {
+        const C b0 = A[off], b1 = A[fs + off], b2 = A[2 * fs + off];
+        B[off] = upd(b0, r0, bdt); B[fs + off] = upd(b1, r1, bdt); B[2 * fs + off] = upd(b2, r2, bdt);
+        const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
+        Cc[off] = upd(c0, r0, adt); Cc[fs + off] = upd(c1, r1, adt); Cc[2 * fs + off] = upd(c2, r2, adt);
+      } else {
+        const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
+        const C n0 = upd(c0, r0, adt), n1 = upd(c1, r1, adt), n2 = upd(c2, r2, adt);
+        A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
+        if (a.phat) ((C*)a.phat)[off] = ph;
+        if (a.partials) {
+          const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+          const double x0 = n0.x, y0 = n0.y, x1 = n1.x, y1 = n1.y, x2 = n2.x, y2 = n2.y;
+          esum += w * (x0 * x0 + y0 * y0 + x1 * x1 + y1 * y1 + x2 * x2 + y2 * y2);
+          const double k0 = K0, k1 = K1, kk2 = K2c;
+          const double ax = k1 * x2 - kk2 * x1, ay = k1 * y2 - kk2 * y1;
+          const double bx = kk2 * x0 - k0 * x2, by = kk2 * y0 - k0 * y2;
+          const double cxv = k0 * x1 - k1 * x0, cyv = k0 * y1 - k1 * y0;
+          osum += w * (ax * ax + ay * ay + bx * bx + by * by + cxv * cxv + cyv * cyv);
+        }
+      }
+    }
+  }
+  if constexpr (MODE == P5_RK3) {
+    if (a.partials) {
+      __shared__ double red[2][TH / 64 + 1];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        esum += __shfl_down(esum, off, 64);
+        osum += __shfl_down(osum, off, 64);
+      }
+      const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+      if (lane == 0) { red[0][wave] = esum; red[1][wave] = osum; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0;
+        for (int w = 0; w < (TH + 63) / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
+        a.partials[2 * blockIdx.x] = s0;
+        a.partials[2 * blockIdx.x + 1] = s1;
+      }
+    }
+  }
+}
+
+}  // namespace sdns

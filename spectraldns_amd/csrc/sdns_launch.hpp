@@ -1,0 +1,50 @@
+// Launcher declarations: one explicit instantiation per working precision
+// lives in sdns_f32.hip / sdns_f64.hip; the C-ABI layer (sdns_abi.hip) calls
+// these with runtime line lengths.  Every launcher returns 0, or -1 when the
+// length has no compiled kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sdns_types.hpp"
+
+// Line lengths with compiled kernels: strided (x, y) complex lines and the
+// half-length complex lines of the z-axis real transforms.
+#define SDNS_STRIDED_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024) X(1536)
+#define SDNS_HALF_SIZES(X) \
+  X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768)
+
+namespace sdns {
+
+template <typename Rt> int launch_p1(int M, const P1Args<Rt>& a, hipStream_t s);
+template <typename Rt> int launch_p2(int M, const P2Args<Rt>& a, hipStream_t s);
+template <typename Rt> int launch_p3(int n, const P3Args<Rt>& a, hipStream_t s);
+// P5: returns the number of blocks in *nblk (for the partial-sum reduction)
+template <typename Rt> int launch_p5(int M, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk);
+// dir = -1 forward, +1 inverse
+template <typename Rt> int launch_strided(int M, int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s);
+// dir = -1 R2C, +1 C2R
+template <typename Rt> int launch_z(int n, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s);
+// columns per block of the strided kernels of length M (0 if unsupported)
+template <typename Rt> int strided_cols(int M);
+
+// elementwise helpers on the spectral grid [3][N0][N1][Nh]
+template <typename Rt> struct EwArgs {
+  void* out;
+  const void* a;
+  const void* b;
+  void* aux;          // pressure output / second output
+  const Rt *kx, *ky, *kz;
+  int N0, N1, Nh, N2;
+  long n;             // points per component
+  Rt nu;
+  int mask_nyquist;
+};
+enum EwOp { EW_CROSS1 = 0, EW_CROSS2 = 1, EW_PRESSURE_DIFFUSION = 2, EW_DIVERGENCE = 3,
+            EW_MUL_MINUS_I = 4, EW_MASK_NYQUIST = 5 };
+template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipStream_t s);
+// FP64 (energy_fourier(u_hat), energy_fourier(curl_hat)) partial sums
+template <typename Rt> int launch_energy(const EwArgs<Rt>& a, double* partials, int nblk, hipStream_t s);
+int launch_sum_partials(const double* partials, int nblk, double* out, double e_scale, double o_scale, hipStream_t s);
+
+}  // namespace sdns

@@ -1,0 +1,252 @@
+// Launcher bodies (size dispatch) and elementwise kernels.  Included once per
+// working precision by sdns_f32.hip / sdns_f64.hip.
+#pragma once
+#include "sdns_kernels.hpp"
+#include "sdns_launch.hpp"
+
+namespace sdns {
+
+template <typename Rt> int launch_p1(int M, const P1Args<Rt>& a, hipStream_t s) {
+  switch (M) {
+#define SDNS_X(m)                                                                       \
+  case m: {                                                                             \
+    using Cfg = SCfg<Rt, m>;                                                            \
+    hipLaunchKernelGGL((k_p1<Rt, m>), dim3((a.Q + Cfg::T - 1) / Cfg::T), dim3(Cfg::THREADS), 0, s, a); \
+    return 0;                                                                           \
+  }
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_p2(int M, const P2Args<Rt>& a, hipStream_t s) {
+  switch (M) {
+#define SDNS_X(m)                                                                       \
+  case m: {                                                                             \
+    using Cfg = SCfg<Rt, m>;                                                            \
+    hipLaunchKernelGGL((k_p2<Rt, m>), dim3((a.Q + Cfg::T - 1) / Cfg::T), dim3(Cfg::THREADS), 0, s, a); \
+    return 0;                                                                           \
+  }
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_p3(int n, const P3Args<Rt>& a, hipStream_t s) {
+  switch (n) {
+#define SDNS_X(m)                                                                       \
+  case m: {                                                                             \
+    using Cfg = ZCfg<Rt, m>;                                                            \
+    hipLaunchKernelGGL((k_p3<Rt, m>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a); \
+    return 0;                                                                           \
+  }
+    SDNS_HALF_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt, int M>
+static void p5_launch_m(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
+  using Cfg = SCfg<Rt, M>;
+  const int nb = (a.Q + Cfg::T - 1) / Cfg::T;
+  if (nblk) *nblk = nb;
+  const dim3 g(nb), b(Cfg::THREADS);
+  switch (mode) {
+    case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, P5_RHS>), g, b, 0, s, a); break;
+    case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK0>), g, b, 0, s, a); break;
+    case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK12>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK3>), g, b, 0, s, a); break;
+  }
+}
+
+template <typename Rt> int launch_p5(int M, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
+  switch (M) {
+#define SDNS_X(m) case m: p5_launch_m<Rt, m>(mode, a, s, nblk); return 0;
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_strided(int M, int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s) {
+  switch (M) {
+#define SDNS_X(m)                                                                                  \
+  case m: {                                                                                        \
+    using Cfg = SCfg<Rt, m>;                                                                       \
+    const dim3 g((a.Q + Cfg::T - 1) / Cfg::T, nfields), b(Cfg::THREADS);                           \
+    if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, m, -1>), g, b, 0, s, a);                        \
+    else hipLaunchKernelGGL((k_strided<Rt, m, 1>), g, b, 0, s, a);                                 \
+    return 0;                                                                                      \
+  }
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_z(int n, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s) {
+  switch (n) {
+#define SDNS_X(m)                                                                                  \
+  case m: {                                                                                        \
+    using Cfg = ZCfg<Rt, m>;                                                                       \
+    const dim3 g((a.L + Cfg::LPB - 1) / Cfg::LPB, nfields), b(Cfg::THREADS);                       \
+    if (dir < 0) hipLaunchKernelGGL((k_z_r2c<Rt, m>), g, b, 0, s, a);                              \
+    else hipLaunchKernelGGL((k_z_c2r<Rt, m>), g, b, 0, s, a);                                      \
+    return 0;                                                                                      \
+  }
+    SDNS_HALF_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int strided_cols(int M) {
+  switch (M) {
+#define SDNS_X(m) case m: return SCfg<Rt, m>::T;
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return 0;
+  }
+}
+
+// ------------------------------------------------------------------ elementwise
+// Spectral point index i -> (ix, iy, kz) of the [N0][N1][Nh] grid.
+template <typename Rt, int OP>
+__global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
+  using C = cx<Rt>;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const long n = a.n;
+  if constexpr (OP == EW_CROSS1) {  // real c = a x b (maths/cross.py:13-25)
+    const Rt* A = (const Rt*)a.a;
+    const Rt* B = (const Rt*)a.b;
+    Rt* O = (Rt*)a.out;
+    const Rt a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
+    const Rt b0 = B[i], b1 = B[n + i], b2 = B[2 * n + i];
+    O[i] = a1 * b2 - a2 * b1;
+    O[n + i] = a2 * b0 - a0 * b2;
+    O[2 * n + i] = a0 * b1 - a1 * b0;
+    return;
+  } else if constexpr (OP == EW_MUL_MINUS_I) {  // out = -1j * a (NS.py:96-99)
+    const C v = ((const C*)a.a)[i];
+    ((C*)a.out)[i] = mk<C>(v.y, -v.x);
+    return;
+  } else {
+    const int kz = (int)(i % a.Nh);
+    const long r = i / a.Nh;
+    const int iy = (int)(r % a.N1);
+    const int ix = (int)(r / a.N1);
+    const Rt K0 = a.kx[ix], K1 = a.ky[iy], K2 = a.kz[kz];
+    const C* A = (const C*)a.a;
+    C* O = (C*)a.out;
+    if constexpr (OP == EW_CROSS2) {  // c = 1j*(K x a) (maths/cross.py:27-32)
+      const C a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
+      const C c0 = mk<C>(K1 * a2.x - K2 * a1.x, K1 * a2.y - K2 * a1.y);
+      const C c1 = mk<C>(K2 * a0.x - K0 * a2.x, K2 * a0.y - K0 * a2.y);
+      const C c2 = mk<C>(K0 * a1.x - K1 * a0.x, K0 * a1.y - K1 * a0.y);
+      O[i] = cmuli<1>(c0);
+      O[n + i] = cmuli<1>(c1);
+      O[2 * n + i] = cmuli<1>(c2);
+    } else if constexpr (OP == EW_DIVERGENCE) {  // 1j*(K . a) (NS.py:106-110)
+      const C a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
+      const C d = mk<C>(K0 * a0.x + K1 * a1.x + K2 * a2.x, K0 * a0.y + K1 * a1.y + K2 * a2.y);
+      O[i] = cmuli<1>(d);
+    } else if constexpr (OP == EW_MASK_NYQUIST) {  // in place on out (NS.py:253-254)
+      if (ix == a.N0 / 2 || iy == a.N1 / 2 || kz == a.N2 / 2) {
+        O[i] = czero<C>();
+        O[n + i] = czero<C>();
+        O[2 * n + i] = czero<C>();
+      }
+    } else if constexpr (OP == EW_PRESSURE_DIFFUSION) {  // NS.py:203-217 on rhs = out, u_hat = a
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2 * K2;
+      const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+      const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2 / den;
+      C r0 = O[i], r1 = O[n + i], r2 = O[2 * n + i];
+      C ph = mk<C>(r0.x * o0 + r1.x * o1, r0.y * o0 + r1.y * o1);
+      ph = mk<C>(ph.x + r2.x * o2, ph.y + r2.y * o2);
+      r0 = mk<C>(r0.x - ph.x * K0, r0.y - ph.y * K0);
+      r1 = mk<C>(r1.x - ph.x * K1, r1.y - ph.y * K1);
+      r2 = mk<C>(r2.x - ph.x * K2, r2.y - ph.y * K2);
+      const Rt z = a.nu * k2;
+      const C s0 = A[i], s1 = A[n + i], s2 = A[2 * n + i];
+      O[i] = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
+      O[n + i] = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
+      O[2 * n + i] = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
+      if (a.aux) ((C*)a.aux)[i] = ph;
+    }
+  }
+}
+
+template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipStream_t s) {
+  const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
+  switch (op) {
+    case EW_CROSS1: hipLaunchKernelGGL((k_elementwise<Rt, EW_CROSS1>), g, b, 0, s, a); return 0;
+    case EW_CROSS2: hipLaunchKernelGGL((k_elementwise<Rt, EW_CROSS2>), g, b, 0, s, a); return 0;
+    case EW_PRESSURE_DIFFUSION: hipLaunchKernelGGL((k_elementwise<Rt, EW_PRESSURE_DIFFUSION>), g, b, 0, s, a); return 0;
+    case EW_DIVERGENCE: hipLaunchKernelGGL((k_elementwise<Rt, EW_DIVERGENCE>), g, b, 0, s, a); return 0;
+    case EW_MUL_MINUS_I: hipLaunchKernelGGL((k_elementwise<Rt, EW_MUL_MINUS_I>), g, b, 0, s, a); return 0;
+    case EW_MASK_NYQUIST: hipLaunchKernelGGL((k_elementwise<Rt, EW_MASK_NYQUIST>), g, b, 0, s, a); return 0;
+    default: return -1;
+  }
+}
+
+// energy_fourier of u_hat and of curl_hat = i K x u_hat, FP64 partial sums per block
+template <typename Rt>
+__global__ void __launch_bounds__(256) k_energy(EwArgs<Rt> a, double* __restrict__ part) {
+  using C = cx<Rt>;
+  const long n = a.n;
+  double es = 0, os = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int kz = (int)(i % a.Nh);
+    const long r = i / a.Nh;
+    const int iy = (int)(r % a.N1);
+    const int ix = (int)(r / a.N1);
+    const double k0 = a.kx[ix], k1 = a.ky[iy], k2 = a.kz[kz];
+    const C* A = (const C*)a.a;
+    const C u0 = A[i], u1 = A[n + i], u2 = A[2 * n + i];
+    const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+    const double x0 = u0.x, y0 = u0.y, x1 = u1.x, y1 = u1.y, x2 = u2.x, y2 = u2.y;
+    es += w * (x0 * x0 + y0 * y0 + x1 * x1 + y1 * y1 + x2 * x2 + y2 * y2);
+    const double ax = k1 * x2 - k2 * x1, ay = k1 * y2 - k2 * y1;
+    const double bx = k2 * x0 - k0 * x2, by = k2 * y0 - k0 * y2;
+    const double cxv = k0 * x1 - k1 * x0, cyv = k0 * y1 - k1 * y0;
+    os += w * (ax * ax + ay * ay + bx * bx + by * by + cxv * cxv + cyv * cyv);
+  }
+  __shared__ double red[2][4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    es += __shfl_down(es, off, 64);
+    os += __shfl_down(os, off, 64);
+  }
+  if (threadIdx.x % 64 == 0) { red[0][threadIdx.x / 64] = es; red[1][threadIdx.x / 64] = os; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+template <typename Rt> int launch_energy(const EwArgs<Rt>& a, double* partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL((k_energy<Rt>), dim3(nblk), dim3(256), 0, s, a, partials);
+  return 0;
+}
+
+}  // namespace sdns
+
+#define SDNS_INSTANTIATE(Rt)                                                                     \
+  namespace sdns {                                                                               \
+  template int launch_p1<Rt>(int, const P1Args<Rt>&, hipStream_t);                               \
+  template int launch_p2<Rt>(int, const P2Args<Rt>&, hipStream_t);                               \
+  template int launch_p3<Rt>(int, const P3Args<Rt>&, hipStream_t);                               \
+  template int launch_p5<Rt>(int, int, const P5Args<Rt>&, hipStream_t, int*);                    \
+  template int launch_strided<Rt>(int, int, int, const StridedArgs<Rt>&, hipStream_t);           \
+  template int launch_z<Rt>(int, int, int, const ZArgs<Rt>&, hipStream_t);                       \
+  template int strided_cols<Rt>(int);                                                            \
+  template int launch_elementwise<Rt>(int, const EwArgs<Rt>&, hipStream_t);                      \
+  template int launch_energy<Rt>(const EwArgs<Rt>&, double*, int, hipStream_t);                  \
+  }

@@ -1,0 +1,122 @@
+"""CPU tests of the host side: configuration semantics (spectralDNS/config.py),
+the solve() loop hook order and dt sequence (spectralDNS/__init__.py:69-123,
+NS.py:112-122), the C-ABI library exports, and the isotropic generator."""
+import ctypes
+import os
+import re
+import types
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_params_semantics():
+    from spectraldns_amd.config import Params
+    p = Params()
+    p.precision = "single"
+    p.M = [4, 5, 6]
+    p.nu = 0.000625
+    p.dt = 0.01
+    p.L = ["2*pi", "4*pi", 3]
+    assert list(p.N) == [16, 32, 64] and not p.M.flags.writeable
+    assert isinstance(p.nu, np.float32) and isinstance(p["nu"], float)
+    assert np.allclose(p.L, [2 * np.pi, 4 * np.pi, 3.0]) and not p.L.flags.writeable
+    p.precision = "double"
+    assert isinstance(p.dt, np.float64)
+    p.N = [8, 8, 8]
+    assert list(p.N) == [8, 8, 8]
+    assert np.allclose(p.dx, p.L / p.N)
+    with pytest.raises(ValueError):
+        p.L = ["__import__('os')", 1, 1]
+
+
+def test_update_and_parse():
+    from spectraldns_amd import config
+    config.update({"nu": 0.002, "dt": 0.005, "T": 0.2, "planner_effort": {"fft": "FFTW_ESTIMATE"}})
+    args = config.triplyperiodic.parse_args(["--M", "5", "5", "5", "--dealias", "3/2-rule", "NS"])
+    assert args.nu == 0.002 and args.dt == 0.005 and args.solver == "NS" and args.dealias == "3/2-rule"
+    assert args.planner_effort["fft"] == "FFTW_ESTIMATE" and args.optimization == "hip"
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1})
+
+
+def test_solve_loop_hook_order_and_dt_sequence():
+    """solve() with a stub solver module: the hooks run in the reference's order
+    and the dt sequence equals the oracle's restatement (last step adjusted)."""
+    import spectraldns_amd as sd
+    from spectraldns_amd import config
+    from spectraldns_amd.solvers import spectralinit
+    from oracle.ns import dt_schedule
+    for prec in ("double", "single"):
+        config.params.update(dict(precision=prec, dt=0.01, T=0.1, t=0.0, tstep=0, make_profile=0,
+                                  convection="Vortex", verbose=False))
+        calls, dts = [], []
+        mod = types.SimpleNamespace()
+        mod.params = config.params
+        mod.Timer = lambda: types.SimpleNamespace(__call__=None, final=lambda v: calls.append("final"))
+
+        class T:
+            def __call__(self):
+                calls.append("timer")
+
+            def final(self, v):
+                calls.append("final")
+        mod.Timer = T
+        mod.profiler = types.SimpleNamespace(getstats=lambda: [], enable=lambda: None)
+        mod.getConvection = lambda c: "conv"
+
+        def getintegrator(rhs, u, solver, context):
+            def f():
+                dts.append(config.params.dt)
+                calls.append("integrate")
+                return u, config.params.dt, config.params.dt
+            return f
+        mod.getintegrator = getintegrator
+        mod.update = lambda c: calls.append("update")
+        mod.end_of_tstep = lambda c: (calls.append("end"), NSend(c))[1]
+        mod.regression_test = lambda c: calls.append("regression")
+        from spectraldns_amd.solvers import NS as _unused  # noqa: F401  (import needs no GPU)
+        NSend = __import__("spectraldns_amd.solvers.NS", fromlist=["end_of_tstep"]).end_of_tstep
+        hfile = types.SimpleNamespace(update=lambda p, **k: calls.append("hdf5"),
+                                      close=lambda: calls.append("close"))
+        ctx = config.AttributeDict(dU=None, u=None, hdf5file=hfile)
+        sd.solve(mod, ctx)
+        assert calls[:5] == ["integrate", "update", "hdf5", "timer", "end"]
+        assert calls[-3:] == ["final", "regression", "close"]
+        ref = dt_schedule(0.1, 0.01, precision=prec)
+        assert len(dts) == len(ref) and all(a == b for a, b in zip(dts, ref))
+        assert config.params.dt == 0.01 or abs(config.params.dt - 0.01) < 1e-9
+
+
+def test_library_exports_every_declared_symbol():
+    from spectraldns_amd import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libsdns_hip.so not built")
+    hdr = open(os.path.join(ROOT, "include", "sdns.h")).read()
+    names = set(re.findall(r"\b(sdns_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 18
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names == set(_abi._SIGS), names ^ set(_abi._SIGS)
+
+
+def test_isotropic_generator_properties():
+    from spectraldns_amd.isotropic import random_solenoidal_field, hermitian_weights
+    from oracle.ns import Grid
+    N = (32, 32, 32)
+    u = random_solenoidal_field(N, seed=7, k0=4.0, energy=0.5)
+    g = Grid(N)
+    assert abs(g.kinetic_energy(u) - 0.5) < 1e-14
+    div = g.K[0] * u[0] + g.K[1] * u[1] + g.K[2] * u[2]
+    assert np.max(abs(div)) < 1e-14
+    assert np.all(u[:, 16] == 0) and np.all(u[:, :, 16] == 0) and np.all(u[..., 16] == 0)
+    assert np.all(u[:, 0, 0, 0] == 0)
+    # Hermitian on kz = 0: the inverse transform is real
+    assert np.max(abs(np.fft.ifft2(u[:, :, :, 0]).imag)) < 1e-15
+    # same seed, same field; spectrum peaks near k0 = 4 (E ~ k^4 exp(-2 (k/4)^2) peaks at k = 4)
+    assert np.array_equal(u, random_solenoidal_field(N, seed=7, k0=4.0, energy=0.5))
+    Ek, bins = g.spectrum(u)
+    assert 3 <= int(np.argmax(Ek)) + 1 <= 5
+    assert list(hermitian_weights(8)) == [1, 2, 2, 2, 1]

@@ -94,12 +94,23 @@ def main():
     ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="first copy a buffer of this size 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     N = (args.N,) * 3
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * 2 ** 20 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     elapsed, passes, diag = gpu_run(N, "single", args.dealias, args.steps, args.warmup)
     ms_step = 1000.0 * elapsed / args.steps
     sps = args.steps / elapsed

@@ -48,6 +48,11 @@ enum { SDNS_MODE_DIRECT = 0, SDNS_MODE_OUTPUT_MASK = 1 };                   /* 2
 int sdns_plan_create(sdns_plan** plan, const int32_t N[3], const double L[3], int precision,
                      int dealias, int mask_nyquist, int dealias_mode, void* hip_stream);
 int sdns_plan_destroy(sdns_plan* plan);
+/* Schedule variant of a pass: pass ids 0..4 as in sdns_timing_read, 5 for the
+ * plain transforms; variant 0 is the default schedule, 1 the alternative
+ * points-per-thread factorisation where one is compiled (otherwise variant 0
+ * runs).  Set by the autotuner (spectraldns_amd/autotune.py). */
+int sdns_plan_set_variant(sdns_plan* plan, int pass, int variant);
 /* Rebind the plan to another HIP stream (NULL = null stream). */
 int sdns_plan_set_stream(sdns_plan* plan, void* hip_stream);
 /* Device workspace the plan needs; sdns_set_workspace hands it caller memory
@@ -100,6 +105,20 @@ int sdns_rk4_steps(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_se
 
 /* Kinetic energy and dissipation of u_hat in FP64 into the device array out[2]. */
 int sdns_energy(sdns_plan* plan, const void* u_hat, double nu, double* out);
+
+/* Large-scale forcing (demo/Isotropic.py:154-184) in place on u_hat: zero the
+ * mean mode, then scale the modes with |K|^2 <= kf^2 by
+ * alpha = sqrt((target - E_upper) / E_lower) so that energy_fourier(u_hat)
+ * equals target; the energies are FP64 reductions and alpha is computed on the
+ * device (no host round trip).  sums (device, 2 doubles, may be NULL) receives
+ * (energy_fourier before scaling, E_lower). */
+int sdns_force(sdns_plan* plan, void* u_hat, double target, double kf, double* sums);
+
+/* Shell sums of the energy spectrum (demo/Isotropic.py:88-115): out (device,
+ * 2*nbins doubles) receives, for shells z = 1..nbins, the sum over modes with
+ * z - 1/2 < |K| <= z + 1/2 of uiui = sum_c w |u_hat_c|^2 (out[z-1]) and the
+ * number of such modes (out[nbins + z - 1]). */
+int sdns_spectrum(sdns_plan* plan, const void* u_hat, int nbins, double* out);
 
 /* Per-pass GPU time from HIP events recorded on the plan's stream around
  * every pass launch (benchmark support).  sdns_timing_enable(plan, 1) starts

@@ -22,6 +22,7 @@ _SIGS = {
                                         ctypes.c_int, ctypes.c_int, _P]),
     "sdns_plan_destroy": (ctypes.c_int, [_P]),
     "sdns_plan_set_stream": (ctypes.c_int, [_P, _P]),
+    "sdns_plan_set_variant": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "sdns_workspace_bytes": (ctypes.c_size_t, [_P]),
     "sdns_set_workspace": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "sdns_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
@@ -37,6 +38,8 @@ _SIGS = {
     "sdns_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                       ctypes.c_double, _P, _P]),
     "sdns_energy": (ctypes.c_int, [_P, _P, ctypes.c_double, _P]),
+    "sdns_force": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, _P]),
+    "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "sdns_last_error": (ctypes.c_char_p, []),
@@ -77,7 +80,8 @@ def _ptr(t):
 class Plan:
     """One box: sizes, dealiasing, precision, twiddles, workspace (NS.py:12-48)."""
 
-    def __init__(self, N, L, precision="double", dealias="2/3-rule", mask_nyquist=True, dealias_mode=0):
+    def __init__(self, N, L, precision="double", dealias="2/3-rule", mask_nyquist=True, dealias_mode=0,
+                 tuned=True):
         import torch
         lib = load()
         if not torch.cuda.is_available():
@@ -110,6 +114,8 @@ class Plan:
         self.workspace_bytes = nbytes
         self.spectral_shape = (self.N[0], self.N[1], self.N[2] // 2 + 1)
         self.physical_shape = self.N
+        from .autotune import apply_schedule
+        self.schedule = apply_schedule(self) if tuned else None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -251,6 +257,30 @@ class Plan:
         self._sync_stream()
         _check(self._lib.sdns_energy(self._h, _ptr(u_hat), float(nu), _ptr(out)))
         return out
+
+    def force(self, u_hat, target, kf, sums=None):
+        """Isotropic.py:154-184 forcing in place (energy_fourier held at target)."""
+        import torch
+        self._spec(u_hat, 3, "u_hat")
+        if sums is not None and (sums.dtype != torch.float64 or sums.numel() < 2 or sums.device.type != "cuda"):
+            raise SdnsError("sums must be a float64 GPU tensor of 2 entries")
+        self._sync_stream()
+        _check(self._lib.sdns_force(self._h, _ptr(u_hat), float(target), float(kf),
+                                    _ptr(sums) if sums is not None else None))
+        return u_hat
+
+    def spectrum_sums(self, u_hat, nbins):
+        """(uiui sums, mode counts) per shell z = 1..nbins, as float64 GPU tensors."""
+        import torch
+        self._spec(u_hat, 3, "u_hat")
+        out = torch.empty(2 * nbins, dtype=torch.float64, device=self.device)
+        self._sync_stream()
+        _check(self._lib.sdns_spectrum(self._h, _ptr(u_hat), int(nbins), _ptr(out)))
+        return out[:nbins], out[nbins:]
+
+    def set_variant(self, pass_id, variant):
+        """Schedule variant (0 default, 1 alternative) of pass 0..4, 5 = plain transforms."""
+        _check(self._lib.sdns_plan_set_variant(self._h, int(pass_id), int(variant)))
 
     # ---------------------------------------------------------------- timing
     PASS_NAMES = ("x_inverse", "y_inverse_curl", "z_cross", "y_forward", "x_forward_rk", "other")

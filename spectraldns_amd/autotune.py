@@ -1,0 +1,93 @@
+"""Schedule autotuner for the pass kernels.
+
+Every pass (0 x-inverse, 1 y-inverse + curl, 2 z + cross product, 3
+y-forward, 4 x-forward + RK stage) is compiled with up to two line-FFT
+schedules (points per thread and the radix sequence that follows from it,
+see sdns_device.hpp).  The tuner times each variant of each pass with HIP
+events inside real RK4 steps of the target box and keeps the fastest; the
+result is stored per box in ``schedules.json`` next to this file and applied
+by ``apply_schedule`` whenever a plan for that box is created.
+
+    python -m spectraldns_amd.autotune --N 256 256 256 --precision single --dealias 3/2-rule
+"""
+import argparse
+import json
+import os
+
+import numpy as np
+
+SCHEDULE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
+PASSES = ("x_inverse", "y_inverse_curl", "z_cross", "y_forward", "x_forward_rk")
+
+
+def key_of(N, precision, dealias, mode=0):
+    return "%dx%dx%d/%s/%s/%d" % (tuple(int(n) for n in N) + (precision, dealias, int(mode)))
+
+
+def load_table(path=SCHEDULE_FILE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def apply_schedule(plan, path=SCHEDULE_FILE):
+    """Set the stored variants for this plan's box (no-op if none stored)."""
+    entry = load_table(path).get(key_of(plan.N, plan.precision, plan.dealias, plan.dealias_mode))
+    if not entry:
+        return None
+    for name, var in entry.get("variants", {}).items():
+        plan.set_variant(PASSES.index(name), int(var))
+    return entry
+
+
+def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, steps=3, seed=1234):
+    """Time both variants of every pass on RK4 steps of a random solenoidal
+    field; return {"variants": {pass: v}, "ms": {pass: [ms_v0, ms_v1]}}."""
+    import torch
+    from ._abi import Plan
+    from .isotropic import random_solenoidal_field
+    plan = Plan(N, L, precision, dealias, True, mode)
+    cd = torch.complex64 if precision == "single" else torch.complex128
+    u0 = torch.from_numpy(random_solenoidal_field(N, L, seed=seed, k0=4.0, energy=0.5)).to(cd).cuda()
+    res = {"variants": {}, "ms": {}}
+    for pid, name in enumerate(PASSES):
+        times = []
+        for var in (0, 1):
+            plan.set_variant(pid, var)
+            u = u0.clone()
+            plan.rk4_steps(u, [1e-3], 1e-3)       # warm-up (first launch of this variant)
+            torch.cuda.synchronize()
+            plan.timing_read()
+            plan.timing_enable(True)
+            plan.rk4_steps(u, [1e-3] * steps, 1e-3)
+            torch.cuda.synchronize()
+            plan.timing_enable(False)
+            ms, n = plan.timing_read()[name]
+            times.append(ms / max(n, 1))
+        best = int(np.argmin(times))
+        plan.set_variant(pid, best)
+        res["variants"][name] = best
+        res["ms"][name] = [round(t, 5) for t in times]
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, nargs=3, default=[256, 256, 256])
+    ap.add_argument("--precision", default="single", choices=("single", "double"))
+    ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule", "None"))
+    ap.add_argument("--mode", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--out", default=SCHEDULE_FILE)
+    args = ap.parse_args()
+    res = tune(tuple(args.N), precision=args.precision, dealias=args.dealias, mode=args.mode, steps=args.steps)
+    table = load_table(args.out)
+    table[key_of(args.N, args.precision, args.dealias, args.mode)] = res
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    print(json.dumps({key_of(args.N, args.precision, args.dealias, args.mode): res}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,8 @@ struct sdns_plan {
   hipStream_t stream = nullptr;
   int in[3][2], out[3][2];
   int nky, nkz, nkyo, nkzo;
+  int nkzp, nkzop;  // kz strides of the work buffers, padded to 16 elements
+  int var[6] = {0, 0, 0, 0, 0, 0};  // schedule variant per pass (see sdns_plan_set_variant)
   // device tables
   std::map<int, void*> tw;       // forward twiddles by line length
   std::map<int, void*> rtw;      // real-transform twiddles by real length
@@ -181,14 +183,14 @@ void compute_sizes(sdns_plan* p) {
   const size_t e = p->esz;
   const size_t M0 = p->M[0], M1 = p->M[1];
   const size_t state = 3ull * p->N[0] * p->N[1] * p->Nh;
-  const size_t w1 = 5ull * M0 * p->nky * p->nkz, w3 = 3ull * M0 * M1 * p->nkzo;
-  const size_t w2 = 6ull * M0 * M1 * p->nkz, w4 = 3ull * M0 * p->nkyo * p->nkzo;
+  const size_t w1 = 5ull * M0 * p->nky * p->nkzp, w3 = 3ull * M0 * M1 * p->nkzop;
+  const size_t w2 = 6ull * M0 * M1 * p->nkzp, w4 = 3ull * M0 * p->nkyo * p->nkzop;
   p->szA = align256(std::max(std::max(w1, w3), state) * e);
   p->szB = align256(std::max(std::max(w2, w4), state) * e);
   p->szS = align256(state * e);
   // P5 blocks: columns N1*Nh over the smallest tile of the supported lengths
   p->max_blocks = p->N[1] * p->Nh;  // upper bound (>= 1 column per block)
-  p->szPart = align256((size_t)std::max(p->max_blocks, 4096) * 2 * sizeof(double));
+  p->szPart = align256((size_t)std::max(p->max_blocks, 4096) * 2 * sizeof(double) + 4096);
 }
 
 int assign_workspace(sdns_plan* p, void* base, size_t bytes) {
@@ -243,21 +245,21 @@ template <typename Rt> int plain_backward(sdns_plan* p, const void* uh, void* up
   StridedArgs<Rt> a{};
   // x inverse: columns (iy, kz) contiguous
   a.in = uh; a.out = p->bufA; a.tw = p->tw[N0];
-  a.Q = (int)plane; a.cdiv = (int)plane; a.in_hi = 0; a.out_hi = 0;
+  a.Q = (int)plane; a.cdiv = (int)plane; a.cvalid = (int)plane; a.in_hi = 0; a.out_hi = 0;
   a.in_row = plane; a.out_row = plane; a.in_field = fs; a.out_field = fs;
   a.rin = rmap(N0 / 2, N0 / 2 - 1, N0); a.rout = a.rin; a.scale = Rt(1);
-  if (launch_strided<Rt>(N0, +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", N0);
+  if (launch_strided<Rt>(N0, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", N0);
   // y inverse: columns (x, kz)
   a.in = p->bufA; a.out = p->bufB; a.tw = p->tw[N1];
-  a.Q = N0 * Nh; a.cdiv = Nh; a.in_hi = plane; a.out_hi = plane; a.in_row = Nh; a.out_row = Nh;
+  a.Q = N0 * Nh; a.cdiv = Nh; a.cvalid = Nh; a.in_hi = plane; a.out_hi = plane; a.in_row = Nh; a.out_row = Nh;
   a.rin = rmap(N1 / 2, N1 / 2 - 1, N1); a.rout = a.rin;
-  if (launch_strided<Rt>(N1, +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", N1);
+  if (launch_strided<Rt>(N1, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", N1);
   // z C2R
   ZArgs<Rt> z{};
   z.in = p->bufB; z.out = up; z.tw = p->tw[N2 / 2]; z.rtw = p->rtw[N2];
   z.L = N0 * N1; z.nk = Nh; z.in_line = Nh; z.in_field = fs; z.out_line = N2 / 2;
   z.out_field = (long)N0 * N1 * (N2 / 2); z.scale = Rt(1);
-  if (launch_z<Rt>(N2 / 2, +1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", N2);
+  if (launch_z<Rt>(N2 / 2, p->var[5], +1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", N2);
   return 0;
 }
 
@@ -268,18 +270,19 @@ template <typename Rt> int plain_forward(sdns_plan* p, const void* up, void* uh,
   z.in = up; z.out = p->bufA; z.tw = p->tw[N2 / 2]; z.rtw = p->rtw[N2];
   z.L = N0 * N1; z.nk = Nh; z.in_line = N2 / 2; z.in_field = (long)N0 * N1 * (N2 / 2);
   z.out_line = Nh; z.out_field = fs; z.scale = Rt(1);
-  if (launch_z<Rt>(N2 / 2, -1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", N2);
+  if (launch_z<Rt>(N2 / 2, p->var[5], -1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", N2);
   StridedArgs<Rt> a{};
   a.in = p->bufA; a.out = p->bufB; a.tw = p->tw[N1];
-  a.Q = N0 * Nh; a.cdiv = Nh; a.in_hi = plane; a.out_hi = plane; a.in_row = Nh; a.out_row = Nh;
+  a.Q = N0 * Nh; a.cdiv = Nh; a.cvalid = Nh; a.in_hi = plane; a.out_hi = plane; a.in_row = Nh; a.out_row = Nh;
   a.in_field = fs; a.out_field = fs;
   a.rin = rmap(N1 / 2, N1 / 2 - 1, N1); a.rout = a.rin; a.scale = Rt(1);
-  if (launch_strided<Rt>(N1, -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", N1);
+  if (launch_strided<Rt>(N1, p->var[5], -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", N1);
   a.in = p->bufB; a.out = uh; a.tw = p->tw[N0];
-  a.Q = (int)plane; a.cdiv = (int)plane; a.in_hi = 0; a.out_hi = 0; a.in_row = plane; a.out_row = plane;
+  a.Q = (int)plane; a.cdiv = (int)plane; a.cvalid = (int)plane; a.in_hi = 0; a.out_hi = 0; a.in_row = plane;
+  a.out_row = plane;
   a.rin = rmap(N0 / 2, N0 / 2 - 1, N0); a.rout = a.rin;
   a.scale = (Rt)(1.0 / ((double)N0 * N1 * N2));
-  if (launch_strided<Rt>(N0, -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", N0);
+  if (launch_strided<Rt>(N0, p->var[5], -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", N0);
   return 0;
 }
 
@@ -290,32 +293,33 @@ template <typename Rt> int conv_passes(sdns_plan* p, const void* u) {
   a1.u = u; a1.w1 = p->bufA; a1.tw = p->tw[M0]; a1.kx = (const Rt*)p->ktab[0];
   a1.N0 = p->N[0]; a1.N1 = p->N[1]; a1.Nh = p->Nh;
   a1.x = rmap(p->in[0][0], p->in[0][1], p->N[0]);
-  a1.y_neg = p->in[1][0]; a1.y_pos = p->in[1][1]; a1.nkz = p->nkz; a1.Q = p->nky * p->nkz;
+  a1.y_neg = p->in[1][0]; a1.y_pos = p->in[1][1]; a1.nkz = p->nkz; a1.nkzp = p->nkzp; a1.Q = p->nky * p->nkzp;
   tstart(p, 0);
-  if (launch_p1<Rt>(M0, a1, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
+  if (launch_p1<Rt>(M0, p->var[0], a1, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
   tstop(p);
   P2Args<Rt> a2{};
   a2.w1 = p->bufA; a2.w2 = p->bufB; a2.tw = p->tw[M1]; a2.ky = (const Rt*)p->ktab[1]; a2.kz = (const Rt*)p->ktab[2];
-  a2.N1 = p->N[1]; a2.M0 = M0; a2.y = rmap(p->in[1][0], p->in[1][1], p->nky); a2.nkz = p->nkz; a2.Q = M0 * p->nkz;
+  a2.N1 = p->N[1]; a2.M0 = M0; a2.y = rmap(p->in[1][0], p->in[1][1], p->nky); a2.nkz = p->nkz; a2.nkzp = p->nkzp;
+  a2.Q = M0 * p->nkzp;
   tstart(p, 1);
-  if (launch_p2<Rt>(M1, a2, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  if (launch_p2<Rt>(M1, p->var[1], a2, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
   tstop(p);
   P3Args<Rt> a3{};
   a3.w2 = p->bufB; a3.w3 = p->bufA; a3.tw = p->tw[p->n2h]; a3.rtw = p->rtw[p->M[2]];
-  a3.nkz = p->nkz; a3.nkzo = p->nkzo; a3.L = M0 * M1;
+  a3.nkz = p->nkz; a3.nkzp = p->nkzp; a3.nkzo = p->nkzo; a3.nkzop = p->nkzop; a3.L = M0 * M1;
   tstart(p, 2);
-  if (launch_p3<Rt>(p->n2h, a3, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", p->M[2]);
+  if (launch_p3<Rt>(p->n2h, p->var[2], a3, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", p->M[2]);
   tstop(p);
   StridedArgs<Rt> a4{};
   a4.in = p->bufA; a4.out = p->bufB; a4.tw = p->tw[M1];
-  a4.Q = M0 * p->nkzo; a4.cdiv = p->nkzo;
-  a4.in_hi = (long)M1 * p->nkzo; a4.in_row = p->nkzo; a4.in_field = (long)M0 * M1 * p->nkzo;
-  a4.out_hi = (long)p->nkyo * p->nkzo; a4.out_row = p->nkzo; a4.out_field = (long)M0 * p->nkyo * p->nkzo;
+  a4.Q = M0 * p->nkzop; a4.cdiv = p->nkzop; a4.cvalid = p->nkzo;
+  a4.in_hi = (long)M1 * p->nkzop; a4.in_row = p->nkzop; a4.in_field = (long)M0 * M1 * p->nkzop;
+  a4.out_hi = (long)p->nkyo * p->nkzop; a4.out_row = p->nkzop; a4.out_field = (long)M0 * p->nkyo * p->nkzop;
   a4.rin = rmap(M1 / 2, M1 - M1 / 2 - 1, M1);
   a4.rout = rmap(p->out[1][0], p->out[1][1], p->nkyo);
   a4.scale = Rt(1);
   tstart(p, 3);
-  if (launch_strided<Rt>(M1, -1, 3, a4, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  if (launch_strided<Rt>(M1, p->var[3], -1, 3, a4, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
   tstop(p);
   return 0;
 }
@@ -326,7 +330,7 @@ template <typename Rt> P5Args<Rt> p5_args(sdns_plan* p, double nu) {
   a.kx = (const Rt*)p->ktab[0]; a.ky = (const Rt*)p->ktab[1]; a.kz = (const Rt*)p->ktab[2];
   a.N0 = p->N[0]; a.N1 = p->N[1]; a.Nh = p->Nh; a.N2 = p->N[2];
   a.x = rmap(p->out[0][0], p->out[0][1], p->N[0]);
-  a.y_neg = p->out[1][0]; a.y_pos = p->out[1][1]; a.nkzo = p->nkzo;
+  a.y_neg = p->out[1][0]; a.y_pos = p->out[1][1]; a.nkzo = p->nkzo; a.nkzop = p->nkzop;
   a.mask_nyquist = p->mask_nyquist;
   a.nu = (Rt)nu;
   a.scale = (Rt)(1.0 / ((double)p->M[0] * p->M[1] * p->M[2]));
@@ -340,7 +344,8 @@ template <typename Rt> int compute_rhs(sdns_plan* p, const void* u, void* rhs, d
   P5Args<Rt> a = p5_args<Rt>(p, nu);
   a.A = const_cast<void*>(u); a.rhs = rhs; a.phat = ph; a.source = src;
   tstart(p, 4);
-  if (launch_p5<Rt>(p->M[0], P5_RHS, a, p->stream, nullptr)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
+  if (launch_p5<Rt>(p->M[0], p->var[4], P5_RHS, a, p->stream, nullptr))
+    return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
   tstop(p);
   return 0;
 }
@@ -350,7 +355,8 @@ template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs) {
   if ((rc = conv_passes<Rt>(p, u))) return rc;
   P5Args<Rt> a = p5_args<Rt>(p, 0.0);
   a.A = const_cast<void*>(u); a.rhs = rhs; a.conv_only = 1;
-  if (launch_p5<Rt>(p->M[0], P5_RHS, a, p->stream, nullptr)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
+  if (launch_p5<Rt>(p->M[0], p->var[4], P5_RHS, a, p->stream, nullptr))
+    return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
   return 0;
 }
 
@@ -374,7 +380,7 @@ template <typename Rt> int rk4_steps(sdns_plan* p, void* u, const double* dts, i
       pa.phat = (rk == 3 && s == nsteps - 1) ? ph : nullptr;
       int nblk = 0;
       tstart(p, 4);
-      if (launch_p5<Rt>(p->M[0], mode, pa, p->stream, &nblk)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
+      if (launch_p5<Rt>(p->M[0], p->var[4], mode, pa, p->stream, &nblk)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", p->M[0]);
       tstop(p);
       if (rk == 3 && diag) {
         tstart(p, 5);
@@ -405,6 +411,30 @@ template <typename Rt> int energy(sdns_plan* p, const void* u, double nu, double
   return 0;
 }
 
+template <typename Rt> int force(sdns_plan* p, void* u, double target, double kf, double* sums) {
+  EwArgs<Rt> a = ew_args<Rt>(p);
+  a.a = u;
+  a.out = u;
+  const Rt kf2 = (Rt)(kf * kf);
+  double* s = sums ? sums : (double*)((char*)p->part + p->szPart - 4096);
+  const int nblk = 1024;
+  launch_force_reduce<Rt>(a, kf2, p->part, nblk, p->stream);
+  if (launch_sum_partials(p->part, nblk, s, 1.0, 1.0, p->stream)) return fail(SDNS_ERR_HIP, "sum");
+  launch_force_scale<Rt>(a, kf2, s, target, p->stream);
+  return 0;
+}
+
+template <typename Rt> int spectrum(sdns_plan* p, const void* u, int nb, double* out) {
+  EwArgs<Rt> a = ew_args<Rt>(p);
+  a.a = u;
+  const size_t avail = p->szPart - 4096;
+  int nblk = (int)std::min<size_t>(256, avail / (2 * (size_t)nb * sizeof(double)));
+  if (nblk < 1) return fail(SDNS_ERR_INVALID, "nbins %d too large", nb);
+  launch_spectrum<Rt>(a, nb, p->part, nblk, p->stream);
+  if (launch_sum_rows(p->part, nblk, 2 * nb, out, p->stream)) return fail(SDNS_ERR_HIP, "sum rows");
+  return 0;
+}
+
 int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SDNS_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
@@ -431,6 +461,19 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__
 }
 int launch_sum_partials(const double* part, int nblk, double* out, double e_scale, double o_scale, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, nblk, out, e_scale, o_scale);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+// out[j] = sum over b of part[b*width + j], fixed order (deterministic).
+__global__ void __launch_bounds__(256) k_sum_rows(const double* __restrict__ part, int nblk, int width,
+                                                  double* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= width) return;
+  double s = 0;
+  for (int b = 0; b < nblk; ++b) s += part[(long)b * width + j];
+  out[j] = s;
+}
+int launch_sum_rows(const double* part, int nblk, int width, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_rows, dim3((width + 255) / 256), dim3(256), 0, s, part, nblk, width, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }  // namespace sdns
@@ -470,6 +513,8 @@ int sdns_plan_create(sdns_plan** plan, const int32_t N[3], const double L[3], in
   p->nkz = p->in[2][1] + 1;
   p->nkyo = p->out[1][0] + p->out[1][1] + 1;
   p->nkzo = p->out[2][1] + 1;
+  p->nkzp = (p->nkz + 15) / 16 * 16;
+  p->nkzop = (p->nkzo + 15) / 16 * 16;
   const int lens[] = {p->M[0], p->M[1], p->N[0], p->N[1]};
   for (int l : lens)
     if (!strided_supported(l)) { delete p; return fail(SDNS_ERR_UNSUPPORTED, "no kernel for line length %d", l); }
@@ -496,6 +541,13 @@ int sdns_plan_destroy(sdns_plan* p) {
     if (p->ktab[i]) (void)hipFree(p->ktab[i]);
   if (p->ws_owned && p->ws) (void)hipFree(p->ws);
   delete p;
+  return 0;
+}
+
+int sdns_plan_set_variant(sdns_plan* p, int pass, int variant) {
+  if (!p) return fail(SDNS_ERR_INVALID, "null plan");
+  if (pass < 0 || pass > 5 || variant < 0 || variant > 1) return fail(SDNS_ERR_INVALID, "pass %d variant %d", pass, variant);
+  p->var[pass] = variant;
   return 0;
 }
 
@@ -599,6 +651,22 @@ int sdns_rk4_steps(sdns_plan* p, void* u, void* u_lo, const double* dts, int nst
   if (rc) return rc;
   rc = p->precision == SDNS_DOUBLE ? rk4_steps<double>(p, u, dts, nsteps, nu, diag, ph)
                                    : rk4_steps<float>(p, u, dts, nsteps, nu, diag, ph);
+  return rc ? rc : check_launch();
+}
+
+int sdns_force(sdns_plan* p, void* u, double target, double kf, double* sums) {
+  if (!p || !u) return fail(SDNS_ERR_INVALID, "bad argument");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = p->precision == SDNS_DOUBLE ? force<double>(p, u, target, kf, sums) : force<float>(p, u, target, kf, sums);
+  return rc ? rc : check_launch();
+}
+
+int sdns_spectrum(sdns_plan* p, const void* u, int nbins, double* out) {
+  if (!p || !u || !out || nbins < 1 || nbins > 8192) return fail(SDNS_ERR_INVALID, "bad argument");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = p->precision == SDNS_DOUBLE ? spectrum<double>(p, u, nbins, out) : spectrum<float>(p, u, nbins, out);
   return rc ? rc : check_launch();
 }
 

@@ -189,29 +189,34 @@ constexpr int num_passes(int M, int E) {
   return n;
 }
 
-// Points per thread for lines along a strided axis (x or y) and for the
-// half-length complex lines of the contiguous z axis.
-constexpr int strided_E(int M) {
+// Points per thread.  Variant 0 is the default schedule; variant 1 is the
+// alternative the autotuner may pick (same as variant 0 where no other
+// factorisation fits).  Strided (x, y) complex lines:
+constexpr int strided_E(int M, int v = 0) {
   switch (M) {
     case 4: return 4;   case 6: return 6;   case 8: return 8;    case 12: return 12;
     case 16: return 16; case 24: return 24; case 32: return 8;   case 36: return 12;
-    case 48: return 12; case 64: return 8;  case 72: return 12;  case 96: return 24;
-    case 128: return 16; case 144: return 12; case 192: return 24; case 256: return 16;
-    case 288: return 12; case 384: return 24; case 512: return 16; case 576: return 12;
-    case 768: return 24; case 1024: return 16; case 1536: return 24;
+    case 48: return 12; case 64: return v ? 16 : 8; case 72: return 12; case 96: return v ? 24 : 12;
+    case 128: return v ? 8 : 16; case 144: return 12; case 192: return v ? 24 : 12; case 256: return v ? 8 : 16;
+    case 288: return 12; case 384: return v ? 24 : 12; case 512: return v ? 8 : 16; case 576: return 12;
+    case 768: return v ? 24 : 12; case 1024: return v ? 8 : 16; case 1536: return v ? 24 : 12;
     default: return 0;
   }
 }
-constexpr int zline_E(int n) {
+// Half-length complex lines of the z-axis real transforms:
+constexpr int zline_E(int n, int v = 0) {
   switch (n) {
     case 4: return 4;   case 6: return 6;   case 8: return 8;   case 12: return 12;
     case 16: return 8;  case 24: return 12; case 32: return 8;  case 36: return 12;
     case 48: return 12; case 64: return 8;  case 72: return 12; case 96: return 12;
-    case 128: return 8; case 144: return 12; case 192: return 12; case 256: return 16;
-    case 288: return 12; case 384: return 12; case 512: return 16; case 768: return 12;
+    case 128: return v ? 16 : 8; case 144: return 12; case 192: return v ? 6 : 12; case 256: return v ? 8 : 16;
+    case 288: return 12; case 384: return v ? 24 : 12; case 512: return v ? 8 : 16; case 768: return 12;
     default: return 0;
   }
 }
+// Number of distinct schedule variants compiled for a length.
+constexpr int strided_variants(int M) { return strided_E(M, 1) != strided_E(M, 0) ? 2 : 1; }
+constexpr int zline_variants(int n) { return zline_E(n, 1) != zline_E(n, 0) ? 2 : 1; }
 
 // tw[m] = exp(-2 pi i m / M), m in [0, M): the forward twiddle table of the
 // line length; the inverse direction uses its conjugate.

@@ -1,4 +1,4 @@
-// Pass kernels of the fused pseudo-spectral RK4 step and their launchers.
+// Pass kernels of the fused pseudo-spectral RK4 step.
 //
 // One RHS evaluation (spectralDNS/solvers/NS.py:219-261, Vortex form
 // NS.py:191-198) is five launches that each stream HBM once:
@@ -11,6 +11,9 @@
 //      stage, the FP64 energy / enstrophy partial sums           [strided, x]
 // Physical space never touches HBM.  Dealiasing is folded into the row maps:
 // truncated modes are neither read nor written, padded bins are zeros.
+//
+// Every kernel takes E (points per thread) as a template parameter so the
+// launcher can pick one of the compiled schedules per line length.
 #pragma once
 #include "sdns_device.hpp"
 #include "sdns_types.hpp"
@@ -18,24 +21,28 @@
 namespace sdns {
 
 // ------------------------------------------------------------------ tiling
-// Columns per block: about 256 threads, LDS tile M*T <= 64 KB, and T*P a
-// multiple of 64 so that every block is made of whole waves (the P5 energy
+// Columns per block: at least 128 contiguous bytes per row (one cache line)
+// and about 256 threads, at most 1024 threads, LDS tile M*T <= 64 KB, and T*P
+// a multiple of 64 so that every block is made of whole waves (the P5 energy
 // reduction shuffles across full 64-lane waves).
 constexpr int strided_T(int M, int P, int esz) {
   int T0 = 256 / P;
+  if (T0 < 128 / esz) T0 = 128 / esz;
+  if (T0 * P > 1024) T0 = 1024 / P;
   if (T0 < 1) T0 = 1;
   if (T0 > 64) T0 = 64;
   for (int T = T0; T >= 1; --T)
     if ((T * P) % 64 == 0 && M * T * esz <= 65536) return T;
   for (int T = 64; T >= 1; --T)
-    if ((T * P) % 64 == 0 && M * T * esz <= 65536) return T;
+    if ((T * P) % 64 == 0 && M * T * esz <= 65536 && T * P <= 1024) return T;
   return 0;
 }
-template <typename Rt, int M> struct SCfg {
-  static constexpr int E = strided_E(M);
+template <typename Rt, int M, int E_> struct SCfg {
+  static constexpr int E = E_;
   static constexpr int P = M / E;
   static constexpr int T = strided_T(M, P, (int)sizeof(cx<Rt>));
   static constexpr int THREADS = T * P;
+  static_assert(T > 0 && THREADS <= 1024, "no tile fits");
 };
 
 constexpr int ZPAD = 1;
@@ -45,8 +52,8 @@ constexpr int z_lpb(int n, int P, int esz) {
   while (L > 1 && L * (n + ZPAD) * esz > 32768) L /= 2;
   return L;
 }
-template <typename Rt, int n> struct ZCfg {
-  static constexpr int E = zline_E(n);
+template <typename Rt, int n, int E_> struct ZCfg {
+  static constexpr int E = E_;
   static constexpr int P = n / E;
   static constexpr int LPB = z_lpb(n, P, (int)sizeof(cx<Rt>));
   static constexpr int THREADS = LPB * P;
@@ -59,137 +66,148 @@ __device__ __forceinline__ int bin_row(int p, int M, RowMap m) {
 }
 template <typename C> __device__ __forceinline__ C czero() { return mk<C>(0, 0); }
 
+// Column tile of this block.  Blocks are dispatched round-robin over the 8
+// XCDs; consecutive tiles go to blocks of the same XCD so that neighbouring
+// tiles, which share 128-byte lines of rows whose length is not a multiple
+// of 16 elements, meet in the same L2.  Launch 8*ceil(ntiles/8) blocks;
+// returns -1 for the padding blocks.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+  const int per = (ntiles + 7) >> 3;
+  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  return t < ntiles ? t : -1;
+}
+inline int xcd_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
 // ------------------------------------------------------------------ P1
-template <typename Rt, int M>
-__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p1(P1Args<Rt> a) {
+// blockIdx.y selects the component: y = 0 -> w (fields 2 and 3 = X(kx w)),
+// y = 1 -> v (fields 1 and 4 = X(kx v)), y = 2 -> u (field 0).
+template <typename Rt, int M, int E>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p1(P1Args<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M>;
-  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using Cfg = SCfg<Rt, M, E>;
+  constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, +1>;
   __shared__ C lds[M * T];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int q = blockIdx.x * T + c;
-  const bool valid = q < a.Q;
-  const int qq = valid ? q : 0;
-  const int iyc = qq / a.nkz, kzi = qq - iyc * a.nkz;
+  const int q = tile * T + c;  // compact column iyc * nkzp + kz
+  const int iyc = q / a.nkzp, kzi = q - iyc * a.nkzp;
+  const bool valid = q < a.Q && kzi < a.nkz;
   const int nky = a.y_neg + a.y_pos + 1;
   const int iy = iyc <= a.y_pos ? iyc : a.N1 - (nky - iyc);
-  const long plane = (long)a.N1 * a.Nh, fs = (long)a.N0 * plane;
-  const C* __restrict__ U = (const C*)a.u + (long)iy * a.Nh + kzi;
+  const int comp = 2 - (int)blockIdx.y;
+  const long plane = (long)a.N1 * a.Nh;
+  const C* __restrict__ U = (const C*)a.u + (long)comp * a.N0 * plane + (valid ? (long)iy * a.Nh + kzi : 0);
   const C* __restrict__ tw = (const C*)a.tw;
-  long src[E];
-  Rt kxe[E];
+  C v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int r = bin_row(t + P * e, M, a.x);
-    src[e] = (valid && r >= 0) ? r * plane : -1;
-    kxe[e] = r >= 0 ? a.kx[r] : Rt(0);
+    v[e] = (valid && r >= 0) ? U[r * plane] : czero<C>();
   }
   C* __restrict__ out = (C*)a.w1 + q;
   const long orow = a.Q, ofield = (long)M * a.Q;
   C* line = lds + c;
-  auto store = [&](int f, C(&v)[E]) {
+  auto store = [&](int f, C(&x)[E]) {
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * orow] = v[e];
+      for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * orow] = x[e];
     }
   };
-  C v[E], x[E];
-  // component 2: X(kx w) -> field 3, X(w) -> field 2
+  if (comp > 0) {
+    C x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[2 * fs + src[e]] : czero<C>();
-#pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = cscale(v[e], kxe[e]);
-  F::run(x, t, line, T, tw);
-  store(3, x);
+    for (int e = 0; e < E; ++e) {
+      const int r = bin_row(t + P * e, M, a.x);
+      x[e] = cscale(v[e], r >= 0 ? a.kx[r] : Rt(0));
+    }
+    F::run(x, t, line, T, tw);
+    store(comp == 2 ? 3 : 4, x);
+  }
   F::run(v, t, line, T, tw);
-  store(2, v);
-  // component 1: X(kx v) -> field 4, X(v) -> field 1
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[fs + src[e]] : czero<C>();
-#pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = cscale(v[e], kxe[e]);
-  F::run(x, t, line, T, tw);
-  store(4, x);
-  F::run(v, t, line, T, tw);
-  store(1, v);
-  // component 0
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? U[src[e]] : czero<C>();
-  F::run(v, t, line, T, tw);
-  store(0, v);
+  store(comp, v);
 }
 
 // ------------------------------------------------------------------ P2
 // curl(u)^ = i (K x u_hat) (maths/cross.py:27-32) assembled after the
 // x-inverse: wx = i(ky W - kz V), wy = i(kz U - X(kx W)), wz = i(X(kx V) - ky U).
-template <typename Rt, int M>
-__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p2(P2Args<Rt> a) {
+// blockIdx.y = 0: V, W -> wx, V, W ; blockIdx.y = 1: U, X(kx W), X(kx V) -> wy, wz, U.
+template <typename Rt, int M, int E>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p2(P2Args<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M>;
-  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using Cfg = SCfg<Rt, M, E>;
+  constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, +1>;
   __shared__ C lds[M * T];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int q = blockIdx.x * T + c;
-  const bool valid = q < a.Q;
-  const int qq = valid ? q : 0;
-  const int x = qq / a.nkz, kzi = qq - x * a.nkz;
+  const int q = tile * T + c;  // column x * nkzp + kz
+  const int x = q / a.nkzp, kzi = q - x * a.nkzp;
+  const bool valid = q < a.Q && kzi < a.nkz;
   const int nky = a.y.base;
-  const long ifield = (long)a.M0 * nky * a.nkz, ofield = (long)a.M0 * M * a.nkz;
-  const C* __restrict__ in = (const C*)a.w1 + (long)x * nky * a.nkz + kzi;
-  C* __restrict__ out = (C*)a.w2 + (long)x * M * a.nkz + kzi;
+  const long ifield = (long)a.M0 * nky * a.nkzp, ofield = (long)a.M0 * M * a.nkzp;
+  const C* __restrict__ in = (const C*)a.w1 + (valid ? (long)x * nky * a.nkzp + kzi : 0);
+  C* __restrict__ out = (C*)a.w2 + (long)x * M * a.nkzp + kzi;
   const C* __restrict__ tw = (const C*)a.tw;
-  const long rs = a.nkz;
-  long src[E];
-  Rt kye[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int p = t + P * e;
-    const int r = bin_row(p, M, a.y);
-    src[e] = (valid && r >= 0) ? r * rs : -1;
-    const int k = p <= a.y.pos ? p : p - M;
-    kye[e] = r >= 0 ? a.ky[k >= 0 ? k : a.N1 + k] : Rt(0);
-  }
-  const Rt kzv = a.kz[kzi];
+  const int rs = a.nkzp;
+  const Rt kzv = a.kz[valid ? kzi : 0];
   C* line = lds + c;
+  auto row = [&](int e) { return valid ? bin_row(t + P * e, M, a.y) : -1; };
+  auto kyv = [&](int e) {
+    const int p = t + P * e;
+    const int k = p <= a.y.pos ? p : p - M;
+    return row(e) >= 0 ? a.ky[k >= 0 ? k : a.N1 + k] : Rt(0);
+  };
+  auto load = [&](int f, C(&v)[E]) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int r = row(e);
+      v[e] = r >= 0 ? in[f * ifield + (long)r * rs] : czero<C>();
+    }
+  };
   auto store = [&](int f, C(&v)[E]) {
     if (valid) {
 #pragma unroll
       for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * rs] = v[e];
     }
   };
-  auto load = [&](int f, C(&v)[E]) {
+  if (blockIdx.y == 0) {
+    C A[E], B[E], om[E];
+    load(1, A);  // V
+    load(2, B);  // W
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = src[e] >= 0 ? in[f * ifield + src[e]] : czero<C>();
-  };
-  C A[E], B[E], om[E];
-  load(1, A);  // V
-  load(2, B);  // W
+    for (int e = 0; e < E; ++e) {
+      const Rt ky = kyv(e);
+      om[e] = cmuli<1>(mk<C>(ky * B[e].x - kzv * A[e].x, ky * B[e].y - kzv * A[e].y));
+    }
+    F::run(om, t, line, T, tw);
+    store(3, om);
+    F::run(A, t, line, T, tw);
+    store(1, A);
+    F::run(B, t, line, T, tw);
+    store(2, B);
+  } else {
+    C A[E], B[E];
+    load(0, A);  // U
+    load(3, B);  // X(kx W)
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    om[e] = cmuli<1>(mk<C>(kye[e] * B[e].x - kzv * A[e].x, kye[e] * B[e].y - kzv * A[e].y));
-  F::run(om, t, line, T, tw);
-  store(3, om);
-  F::run(A, t, line, T, tw);
-  store(1, A);
-  F::run(B, t, line, T, tw);
-  store(2, B);
-  load(0, A);  // U
-  load(4, B);  // X(kx V)
+    for (int e = 0; e < E; ++e)
+      B[e] = cmuli<1>(mk<C>(kzv * A[e].x - B[e].x, kzv * A[e].y - B[e].y));
+    F::run(B, t, line, T, tw);
+    store(4, B);
+    load(4, B);  // X(kx V)
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    om[e] = cmuli<1>(mk<C>(B[e].x - kye[e] * A[e].x, B[e].y - kye[e] * A[e].y));
-  F::run(om, t, line, T, tw);
-  store(5, om);
-  load(3, B);  // X(kx W)
-#pragma unroll
-  for (int e = 0; e < E; ++e)
-    om[e] = cmuli<1>(mk<C>(kzv * A[e].x - B[e].x, kzv * A[e].y - B[e].y));
-  F::run(om, t, line, T, tw);
-  store(4, om);
-  F::run(A, t, line, T, tw);
-  store(0, A);
+    for (int e = 0; e < E; ++e) {
+      const Rt ky = kyv(e);
+      B[e] = cmuli<1>(mk<C>(B[e].x - ky * A[e].x, B[e].y - ky * A[e].y));
+    }
+    F::run(B, t, line, T, tw);
+    store(5, B);
+    F::run(A, t, line, T, tw);
+    store(0, A);
+  }
 }
 
 // ------------------------------------------------------------------ z-line real transforms
@@ -199,17 +217,18 @@ __global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p2(P2Args<Rt> a) {
 //       as numpy/FFTW c2r do)
 //  R2C: Z = FFT_n(x[2m] + i x[2m+1]),
 //       X[k] = (Z[k] + conj Z[n-k])/2 - (i/2) e^{-2 pi i k/M2} (Z[k] - conj Z[n-k]).
+
+// Z[t + P e] from a spectral line staged in LDS (my[0..n], zero above nk).
 template <typename Rt, int n, int E>
-__device__ __forceinline__ void c2r_pre(cx<Rt> (&z)[E], const cx<Rt>* __restrict__ in, int nk, bool valid,
-                                        const cx<Rt>* __restrict__ rtw, int t) {
+__device__ __forceinline__ void c2r_pre_lds(cx<Rt> (&z)[E], const cx<Rt>* my, const cx<Rt>* __restrict__ rtw,
+                                            int t) {
   using C = cx<Rt>;
   constexpr int P = n / E;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = t + P * e;
-    const int pn = n - p;
-    C xa = (valid && p < nk) ? in[p] : czero<C>();
-    C xb = (valid && pn < nk) ? in[pn] : czero<C>();
+    C xa = my[p];
+    C xb = my[n - p];
     if (p == 0) { xa.y = 0; xb.y = 0; }
     const C ev = mk<C>(xa.x + xb.x, xa.y - xb.y);
     const C d = mk<C>(xa.x - xb.x, xa.y + xb.y);
@@ -244,12 +263,35 @@ __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __r
   __syncthreads();
 }
 
+// Staged load of one spectral line: the P threads of a line read bins
+// t, t+P, ... (coalesced) into registers; stage_store puts them into LDS with
+// zeros above nk (up to the Nyquist bin n).
+template <typename Rt, int n, int P>
+struct LineStage {
+  static constexpr int K = (n + 1 + P - 1) / P;
+  cx<Rt> r[K];
+  __device__ __forceinline__ void load(const cx<Rt>* __restrict__ in, int nk, bool valid, int t) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = t + P * k;
+      r[k] = (valid && i < nk) ? in[i] : czero<cx<Rt>>();
+    }
+  }
+  __device__ __forceinline__ void store(cx<Rt>* my, int t) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = t + P * k;
+      if (i <= n) my[i] = r[k];
+    }
+  }
+};
+
 // ------------------------------------------------------------------ P3
-template <typename Rt, int n>
-__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_p3(P3Args<Rt> a) {
+template <typename Rt, int n, int E>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_p3(P3Args<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = ZCfg<Rt, n>;
-  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Cfg = ZCfg<Rt, n, E>;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Fi = LineFFT<Rt, n, E, +1>;
   using Ff = LineFFT<Rt, n, E, -1>;
   __shared__ C lds[LPB * (n + ZPAD)];
@@ -260,39 +302,45 @@ __global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_p3(P3Args<Rt> a) {
   C* my = lds + l * (n + ZPAD);
   const C* __restrict__ tw = (const C*)a.tw;
   const C* __restrict__ rtw = (const C*)a.rtw;
-  const C* __restrict__ in = (const C*)a.w2 + li * a.nkz;
-  const long ifield = (long)a.L * a.nkz;
+  const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
+  const long ifield = (long)a.L * a.nkzp;
+  LineStage<Rt, n, P> st;
+  st.load(in, a.nkz, valid, t);
   C ph[6][E];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
-    c2r_pre<Rt, n, E>(ph[f], in + f * ifield, a.nkz, valid, rtw, t);
+    st.store(my, t);
+    __syncthreads();
+    c2r_pre_lds<Rt, n, E>(ph[f], my, rtw, t);
+    __syncthreads();
+    if (f < 5) st.load(in + (f + 1) * ifield, a.nkz, valid, t);  // next field in flight during the FFT
     Fi::run(ph[f], t, my, 1, tw);
   }
-  // u x w on the packed points (real part: x[2m], imaginary part: x[2m+1])
-  C cr[3][E];
+  // u x w on the packed points (real part: x[2m], imaginary part: x[2m+1]);
+  // the products overwrite the velocity registers
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const C u0 = ph[0][e], u1 = ph[1][e], u2 = ph[2][e];
     const C w0 = ph[3][e], w1 = ph[4][e], w2 = ph[5][e];
-    cr[0][e] = mk<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-    cr[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-    cr[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+    ph[0][e] = mk<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+    ph[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+    ph[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
   }
-  C* __restrict__ out = (C*)a.w3 + li * a.nkzo;
-  const long ofield = (long)a.L * a.nkzo;
+  C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
+  const long ofield = (long)a.L * a.nkzop;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    Ff::run(cr[i], t, my, 1, tw);
-    r2c_post<Rt, n, E>(cr[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+    Ff::run(ph[i], t, my, 1, tw);
+    r2c_post<Rt, n, E>(ph[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
   }
 }
 
 // ------------------------------------------------------------------ plain z transforms
-template <typename Rt, int n>
-__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
+template <typename Rt, int n, int E>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = ZCfg<Rt, n>;
-  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Cfg = ZCfg<Rt, n, E>;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Fi = LineFFT<Rt, n, E, +1>;
   __shared__ C lds[LPB * (n + ZPAD)];
   const int t = threadIdx.x % P, l = threadIdx.x / P;
@@ -301,8 +349,13 @@ __global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
   const long li = valid ? line : 0;
   C* my = lds + l * (n + ZPAD);
   const C* in = (const C*)a.in + blockIdx.y * a.in_field + li * a.in_line;
+  LineStage<Rt, n, P> st;
+  st.load(in, a.nk, valid, t);
+  st.store(my, t);
+  __syncthreads();
   C z[E];
-  c2r_pre<Rt, n, E>(z, in, a.nk, valid, (const C*)a.rtw, t);
+  c2r_pre_lds<Rt, n, E>(z, my, (const C*)a.rtw, t);
+  __syncthreads();
   Fi::run(z, t, my, 1, (const C*)a.tw);
   if (valid) {
     C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;  // real line viewed as n complex pairs
@@ -311,11 +364,11 @@ __global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
   }
 }
 
-template <typename Rt, int n>
-__global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_r2c(ZArgs<Rt> a) {
+template <typename Rt, int n, int E>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_r2c(ZArgs<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = ZCfg<Rt, n>;
-  constexpr int E = Cfg::E, P = Cfg::P, LPB = Cfg::LPB;
+  using Cfg = ZCfg<Rt, n, E>;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Ff = LineFFT<Rt, n, E, -1>;
   __shared__ C lds[LPB * (n + ZPAD)];
   const int t = threadIdx.x % P, l = threadIdx.x / P;
@@ -333,19 +386,20 @@ __global__ void __launch_bounds__((ZCfg<Rt, n>::THREADS)) k_z_r2c(ZArgs<Rt> a) {
 }
 
 // ------------------------------------------------------------------ generic strided
-template <typename Rt, int M, int DIR>
-__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_strided(StridedArgs<Rt> a) {
+template <typename Rt, int M, int E, int DIR>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_strided(StridedArgs<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M>;
-  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T;
+  using Cfg = SCfg<Rt, M, E>;
+  constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, DIR>;
   __shared__ C lds[M * T];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int q = blockIdx.x * T + c;
-  const bool valid = q < a.Q;
-  const int qq = valid ? q : 0;
-  const int hi = qq / a.cdiv, lo = qq - hi * a.cdiv;
-  const C* __restrict__ in = (const C*)a.in + blockIdx.y * a.in_field + hi * a.in_hi + lo;
+  const int q = tile * T + c;
+  const int hi = q / a.cdiv, lo = q - hi * a.cdiv;
+  const bool valid = q < a.Q && lo < a.cvalid;
+  const C* __restrict__ in = (const C*)a.in + blockIdx.y * a.in_field + (valid ? hi * a.in_hi + lo : 0);
   C* __restrict__ out = (C*)a.out + blockIdx.y * a.out_field + hi * a.out_hi + lo;
   C v[E];
 #pragma unroll
@@ -364,17 +418,18 @@ __global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_strided(StridedArgs<
 }
 
 // ------------------------------------------------------------------ P5
-template <int MODE> struct P5Tag {};
-
-template <typename Rt, int M, int MODE>
-__global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p5(P5Args<Rt> a) {
+template <typename Rt, int M, int E, int MODE>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p5(P5Args<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M>;
-  constexpr int E = Cfg::E, P = Cfg::P, T = Cfg::T, TH = Cfg::THREADS;
+  using Cfg = SCfg<Rt, M, E>;
+  constexpr int P = Cfg::P, T = Cfg::T, TH = Cfg::THREADS;
   using F = LineFFT<Rt, M, E, -1>;
   __shared__ C lds[M * T];
+  const int ntiles = (a.Q + T - 1) / T;
+  const int tile = xcd_tile(ntiles);
+  if (tile < 0) return;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
-  const int q = blockIdx.x * T + c;
+  const int q = tile * T + c;
   const bool valid = q < a.Q;
   const int qq = valid ? q : 0;
   const int iy = qq / a.Nh, kz = qq - iy * a.Nh;
@@ -382,8 +437,8 @@ __global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p5(P5Args<Rt> a) {
   const int nkyo = a.y_neg + a.y_pos + 1;
   const bool colconv = valid && ky >= -a.y_neg && ky <= a.y_pos && kz < a.nkzo;
   const int kyc = ky >= 0 ? ky : nkyo + ky;
-  const long wrow = (long)nkyo * a.nkzo, wfield = (long)M * wrow;
-  const C* __restrict__ w4 = (const C*)a.w4 + (colconv ? (long)kyc * a.nkzo + kz : 0);
+  const long wrow = (long)nkyo * a.nkzop, wfield = (long)M * wrow;
+  const C* __restrict__ w4 = (const C*)a.w4 + (colconv ? (long)kyc * a.nkzop + kz : 0);
   const C* __restrict__ tw = (const C*)a.tw;
   C X0[E], X1[E], X2[E];
 #pragma unroll
@@ -492,7 +547,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p5(P5Args<Rt> a) {
   }
   if constexpr (MODE == P5_RK3) {
     if (a.partials) {
-      __shared__ double red[2][TH / 64 + 1];
+      __shared__ double red[2][TH / 64];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         esum += __shfl_down(esum, off, 64);
@@ -503,9 +558,9 @@ __global__ void __launch_bounds__((SCfg<Rt, M>::THREADS)) k_p5(P5Args<Rt> a) {
       __syncthreads();
       if (threadIdx.x == 0) {
         double s0 = 0, s1 = 0;
-        for (int w = 0; w < (TH + 63) / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
-        a.partials[2 * blockIdx.x] = s0;
-        a.partials[2 * blockIdx.x + 1] = s1;
+        for (int w = 0; w < TH / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
+        a.partials[2 * tile] = s0;
+        a.partials[2 * tile + 1] = s1;
       }
     }
   }

@@ -16,17 +16,18 @@
 
 namespace sdns {
 
-template <typename Rt> int launch_p1(int M, const P1Args<Rt>& a, hipStream_t s);
-template <typename Rt> int launch_p2(int M, const P2Args<Rt>& a, hipStream_t s);
-template <typename Rt> int launch_p3(int n, const P3Args<Rt>& a, hipStream_t s);
-// P5: returns the number of blocks in *nblk (for the partial-sum reduction)
-template <typename Rt> int launch_p5(int M, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk);
+// `var` selects the compiled schedule (points per thread) of the length;
+// variants that do not exist for a length fall back to variant 0.
+template <typename Rt> int launch_p1(int M, int var, const P1Args<Rt>& a, hipStream_t s);
+template <typename Rt> int launch_p2(int M, int var, const P2Args<Rt>& a, hipStream_t s);
+template <typename Rt> int launch_p3(int n, int var, const P3Args<Rt>& a, hipStream_t s);
+// P5: returns the number of column tiles in *nblk (entries of the partial sums)
+template <typename Rt> int launch_p5(int M, int var, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk);
 // dir = -1 forward, +1 inverse
-template <typename Rt> int launch_strided(int M, int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s);
+template <typename Rt> int launch_strided(int M, int var, int dir, int nfields, const StridedArgs<Rt>& a,
+                                          hipStream_t s);
 // dir = -1 R2C, +1 C2R
-template <typename Rt> int launch_z(int n, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s);
-// columns per block of the strided kernels of length M (0 if unsupported)
-template <typename Rt> int strided_cols(int M);
+template <typename Rt> int launch_z(int n, int var, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s);
 
 // elementwise helpers on the spectral grid [3][N0][N1][Nh]
 template <typename Rt> struct EwArgs {
@@ -46,5 +47,15 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
 // FP64 (energy_fourier(u_hat), energy_fourier(curl_hat)) partial sums
 template <typename Rt> int launch_energy(const EwArgs<Rt>& a, double* partials, int nblk, hipStream_t s);
 int launch_sum_partials(const double* partials, int nblk, double* out, double e_scale, double o_scale, hipStream_t s);
+// Large-scale forcing (demo/Isotropic.py:154-184): partial sums of (energy_fourier(u),
+// energy_fourier(u on |K|^2 <= kf^2)) excluding the mean mode, then the in-place rescale of the
+// low modes by alpha = sqrt((target - upper)/lower) read from sums[0..1] on the device.
+template <typename Rt> int launch_force_reduce(const EwArgs<Rt>& a, Rt kf2, double* partials, int nblk, hipStream_t s);
+template <typename Rt> int launch_force_scale(const EwArgs<Rt>& a, Rt kf2, const double* sums, double target,
+                                              hipStream_t s);
+// Shell spectrum (demo/Isotropic.py:88-115): per-block partial sums of uiui and mode counts for
+// shells z = 1..nb (z = #{bins b_i = i + 1/2 : b_i < |K|}), out layout [nblk][2][nb].
+template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* partials, int nblk, hipStream_t s);
+int launch_sum_rows(const double* partials, int nblk, int width, double* out, hipStream_t s);
 
 }  // namespace sdns

@@ -6,109 +6,124 @@
 
 namespace sdns {
 
-template <typename Rt> int launch_p1(int M, const P1Args<Rt>& a, hipStream_t s) {
-  switch (M) {
-#define SDNS_X(m)                                                                       \
-  case m: {                                                                             \
-    using Cfg = SCfg<Rt, m>;                                                            \
-    hipLaunchKernelGGL((k_p1<Rt, m>), dim3((a.Q + Cfg::T - 1) / Cfg::T), dim3(Cfg::THREADS), 0, s, a); \
-    return 0;                                                                           \
-  }
-    SDNS_STRIDED_SIZES(SDNS_X)
-#undef SDNS_X
-    default: return -1;
-  }
+// ---------------------------------------------------------------- launch helpers
+template <typename Rt, int M, int E> void p1_go(const P1Args<Rt>& a, hipStream_t s) {
+  using Cfg = SCfg<Rt, M, E>;
+  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
+  hipLaunchKernelGGL((k_p1<Rt, M, E>), dim3(xcd_grid(nt), 3), dim3(Cfg::THREADS), 0, s, a);
 }
-
-template <typename Rt> int launch_p2(int M, const P2Args<Rt>& a, hipStream_t s) {
-  switch (M) {
-#define SDNS_X(m)                                                                       \
-  case m: {                                                                             \
-    using Cfg = SCfg<Rt, m>;                                                            \
-    hipLaunchKernelGGL((k_p2<Rt, m>), dim3((a.Q + Cfg::T - 1) / Cfg::T), dim3(Cfg::THREADS), 0, s, a); \
-    return 0;                                                                           \
-  }
-    SDNS_STRIDED_SIZES(SDNS_X)
-#undef SDNS_X
-    default: return -1;
-  }
+template <typename Rt, int M, int E> void p2_go(const P2Args<Rt>& a, hipStream_t s) {
+  using Cfg = SCfg<Rt, M, E>;
+  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
+  hipLaunchKernelGGL((k_p2<Rt, M, E>), dim3(xcd_grid(nt), 2), dim3(Cfg::THREADS), 0, s, a);
 }
-
-template <typename Rt> int launch_p3(int n, const P3Args<Rt>& a, hipStream_t s) {
-  switch (n) {
-#define SDNS_X(m)                                                                       \
-  case m: {                                                                             \
-    using Cfg = ZCfg<Rt, m>;                                                            \
-    hipLaunchKernelGGL((k_p3<Rt, m>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a); \
-    return 0;                                                                           \
-  }
-    SDNS_HALF_SIZES(SDNS_X)
-#undef SDNS_X
-    default: return -1;
-  }
+template <typename Rt, int n, int E> void p3_go(const P3Args<Rt>& a, hipStream_t s) {
+  using Cfg = ZCfg<Rt, n, E>;
+  hipLaunchKernelGGL((k_p3<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
 }
-
-template <typename Rt, int M>
-static void p5_launch_m(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
-  using Cfg = SCfg<Rt, M>;
-  const int nb = (a.Q + Cfg::T - 1) / Cfg::T;
-  if (nblk) *nblk = nb;
-  const dim3 g(nb), b(Cfg::THREADS);
+template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
+  using Cfg = SCfg<Rt, M, E>;
+  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
+  if (nblk) *nblk = nt;
+  const dim3 g(xcd_grid(nt)), b(Cfg::THREADS);
   switch (mode) {
-    case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, P5_RHS>), g, b, 0, s, a); break;
-    case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK0>), g, b, 0, s, a); break;
-    case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK12>), g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_p5<Rt, M, P5_RK3>), g, b, 0, s, a); break;
+    case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RHS>), g, b, 0, s, a); break;
+    case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK0>), g, b, 0, s, a); break;
+    case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK12>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK3>), g, b, 0, s, a); break;
   }
 }
+template <typename Rt, int M, int E> void strided_go(int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s) {
+  using Cfg = SCfg<Rt, M, E>;
+  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
+  const dim3 g(xcd_grid(nt), nfields), b(Cfg::THREADS);
+  if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, M, E, -1>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_strided<Rt, M, E, 1>), g, b, 0, s, a);
+}
+template <typename Rt, int n, int E> void z_go(int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s) {
+  using Cfg = ZCfg<Rt, n, E>;
+  const dim3 g((a.L + Cfg::LPB - 1) / Cfg::LPB, nfields), b(Cfg::THREADS);
+  if (dir < 0) hipLaunchKernelGGL((k_z_r2c<Rt, n, E>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_z_c2r<Rt, n, E>), g, b, 0, s, a);
+}
 
-template <typename Rt> int launch_p5(int M, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
+// (variant 1 of a length with a single schedule instantiates the same kernel as variant 0)
+template <typename Rt> int launch_p1(int M, int var, const P1Args<Rt>& a, hipStream_t s) {
   switch (M) {
-#define SDNS_X(m) case m: p5_launch_m<Rt, m>(mode, a, s, nblk); return 0;
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) p1_go<Rt, m, strided_E(m, 1)>(a, s);                        \
+    else p1_go<Rt, m, strided_E(m, 0)>(a, s);                            \
+    return 0;
     SDNS_STRIDED_SIZES(SDNS_X)
 #undef SDNS_X
     default: return -1;
   }
 }
 
-template <typename Rt> int launch_strided(int M, int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s) {
+template <typename Rt> int launch_p2(int M, int var, const P2Args<Rt>& a, hipStream_t s) {
   switch (M) {
-#define SDNS_X(m)                                                                                  \
-  case m: {                                                                                        \
-    using Cfg = SCfg<Rt, m>;                                                                       \
-    const dim3 g((a.Q + Cfg::T - 1) / Cfg::T, nfields), b(Cfg::THREADS);                           \
-    if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, m, -1>), g, b, 0, s, a);                        \
-    else hipLaunchKernelGGL((k_strided<Rt, m, 1>), g, b, 0, s, a);                                 \
-    return 0;                                                                                      \
-  }
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) p2_go<Rt, m, strided_E(m, 1)>(a, s);                        \
+    else p2_go<Rt, m, strided_E(m, 0)>(a, s);                            \
+    return 0;
     SDNS_STRIDED_SIZES(SDNS_X)
 #undef SDNS_X
     default: return -1;
   }
 }
 
-template <typename Rt> int launch_z(int n, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s) {
+template <typename Rt> int launch_p3(int n, int var, const P3Args<Rt>& a, hipStream_t s) {
   switch (n) {
-#define SDNS_X(m)                                                                                  \
-  case m: {                                                                                        \
-    using Cfg = ZCfg<Rt, m>;                                                                       \
-    const dim3 g((a.L + Cfg::LPB - 1) / Cfg::LPB, nfields), b(Cfg::THREADS);                       \
-    if (dir < 0) hipLaunchKernelGGL((k_z_r2c<Rt, m>), g, b, 0, s, a);                              \
-    else hipLaunchKernelGGL((k_z_c2r<Rt, m>), g, b, 0, s, a);                                      \
-    return 0;                                                                                      \
-  }
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) p3_go<Rt, m, zline_E(m, 1)>(a, s);                          \
+    else p3_go<Rt, m, zline_E(m, 0)>(a, s);                              \
+    return 0;
     SDNS_HALF_SIZES(SDNS_X)
 #undef SDNS_X
     default: return -1;
   }
 }
 
-template <typename Rt> int strided_cols(int M) {
+template <typename Rt> int launch_p5(int M, int var, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
   switch (M) {
-#define SDNS_X(m) case m: return SCfg<Rt, m>::T;
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) p5_go<Rt, m, strided_E(m, 1)>(mode, a, s, nblk);            \
+    else p5_go<Rt, m, strided_E(m, 0)>(mode, a, s, nblk);                \
+    return 0;
     SDNS_STRIDED_SIZES(SDNS_X)
 #undef SDNS_X
-    default: return 0;
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_strided(int M, int var, int dir, int nfields, const StridedArgs<Rt>& a,
+                                          hipStream_t s) {
+  switch (M) {
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) strided_go<Rt, m, strided_E(m, 1)>(dir, nfields, a, s);     \
+    else strided_go<Rt, m, strided_E(m, 0)>(dir, nfields, a, s);         \
+    return 0;
+    SDNS_STRIDED_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
+  }
+}
+
+template <typename Rt> int launch_z(int n, int var, int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s) {
+  switch (n) {
+#define SDNS_X(m)                                                        \
+  case m:                                                                \
+    if (var) z_go<Rt, m, zline_E(m, 1)>(dir, nfields, a, s);             \
+    else z_go<Rt, m, zline_E(m, 0)>(dir, nfields, a, s);                 \
+    return 0;
+    SDNS_HALF_SIZES(SDNS_X)
+#undef SDNS_X
+    default: return -1;
   }
 }
 
@@ -236,17 +251,133 @@ template <typename Rt> int launch_energy(const EwArgs<Rt>& a, double* partials, 
   return 0;
 }
 
+// ------------------------------------------------------------------ forcing
+template <typename Rt>
+__global__ void __launch_bounds__(256) k_force_reduce(EwArgs<Rt> a, Rt kf2, double* __restrict__ part) {
+  using C = cx<Rt>;
+  const long n = a.n;
+  double s_all = 0, s_low = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    if (i == 0) continue;  // the mean mode is set to zero first (Isotropic.py:163-164)
+    const int kz = (int)(i % a.Nh);
+    const long r = i / a.Nh;
+    const int iy = (int)(r % a.N1);
+    const int ix = (int)(r / a.N1);
+    const Rt K0 = a.kx[ix], K1 = a.ky[iy], K2 = a.kz[kz];
+    Rt k2 = K0 * K0;
+    k2 += K1 * K1;
+    k2 += K2 * K2;
+    const C* A = (const C*)a.a;
+    const C u0 = A[i], u1 = A[n + i], u2 = A[2 * n + i];
+    const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+    const double e = w * ((double)u0.x * u0.x + (double)u0.y * u0.y + (double)u1.x * u1.x + (double)u1.y * u1.y +
+                          (double)u2.x * u2.x + (double)u2.y * u2.y);
+    s_all += e;
+    if (k2 <= kf2) s_low += e;
+  }
+  __shared__ double red[2][4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s_all += __shfl_down(s_all, off, 64);
+    s_low += __shfl_down(s_low, off, 64);
+  }
+  if (threadIdx.x % 64 == 0) { red[0][threadIdx.x / 64] = s_all; red[1][threadIdx.x / 64] = s_low; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+template <typename Rt>
+__global__ void __launch_bounds__(256) k_force_scale(EwArgs<Rt> a, Rt kf2, const double* __restrict__ sums,
+                                                    double target) {
+  using C = cx<Rt>;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const long n = a.n;
+  C* U = (C*)a.out;
+  if (i == 0) {
+    U[0] = czero<C>(); U[n] = czero<C>(); U[2 * n] = czero<C>();
+    return;
+  }
+  const int kz = (int)(i % a.Nh);
+  const long r = i / a.Nh;
+  const int iy = (int)(r % a.N1);
+  const int ix = (int)(r / a.N1);
+  const Rt K0 = a.kx[ix], K1 = a.ky[iy], K2 = a.kz[kz];
+  Rt k2 = K0 * K0;
+  k2 += K1 * K1;
+  k2 += K2 * K2;
+  if (!(k2 <= kf2)) return;
+  const double lower = sums[1], upper = sums[0] - sums[1];
+  const Rt alpha = (Rt)sqrt((target - upper) / lower);
+  U[i] = cscale(U[i], alpha);
+  U[n + i] = cscale(U[n + i], alpha);
+  U[2 * n + i] = cscale(U[2 * n + i], alpha);
+}
+
+template <typename Rt> int launch_force_reduce(const EwArgs<Rt>& a, Rt kf2, double* partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL((k_force_reduce<Rt>), dim3(nblk), dim3(256), 0, s, a, kf2, partials);
+  return 0;
+}
+template <typename Rt> int launch_force_scale(const EwArgs<Rt>& a, Rt kf2, const double* sums, double target,
+                                              hipStream_t s) {
+  hipLaunchKernelGGL((k_force_scale<Rt>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a, kf2, sums, target);
+  return 0;
+}
+
+// ------------------------------------------------------------------ spectrum
+template <typename Rt>
+__global__ void __launch_bounds__(256) k_spectrum(EwArgs<Rt> a, int nb, double* __restrict__ part) {
+  using C = cx<Rt>;
+  extern __shared__ double hist[];  // [2][nb]: uiui sums, counts
+  for (int b = threadIdx.x; b < 2 * nb; b += 256) hist[b] = 0.0;
+  __syncthreads();
+  const long n = a.n;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int kz = (int)(i % a.Nh);
+    const long r = i / a.Nh;
+    const int iy = (int)(r % a.N1);
+    const int ix = (int)(r / a.N1);
+    const Rt K0 = a.kx[ix], K1 = a.ky[iy], K2 = a.kz[kz];
+    Rt k2 = K0 * K0;
+    k2 += K1 * K1;
+    k2 += K2 * K2;
+    const double x = (double)sqrt(k2);
+    // np.digitize(x, bins=[0.5, 1.5, ...], right=True): number of bins strictly below x
+    int z = x > 0.5 ? (int)ceil(x - 0.5) : 0;
+    if (z < 1 || z > nb) continue;
+    const C* A = (const C*)a.a;
+    const C u0 = A[i], u1 = A[n + i], u2 = A[2 * n + i];
+    const double w = (kz == 0 || kz == a.Nh - 1) ? 1.0 : 2.0;
+    const double e = w * ((double)u0.x * u0.x + (double)u0.y * u0.y + (double)u1.x * u1.x + (double)u1.y * u1.y +
+                          (double)u2.x * u2.x + (double)u2.y * u2.y);
+    atomicAdd(&hist[z - 1], e);
+    atomicAdd(&hist[nb + z - 1], 1.0);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 2 * nb; b += 256) part[(long)blockIdx.x * 2 * nb + b] = hist[b];
+}
+
+template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL((k_spectrum<Rt>), dim3(nblk), dim3(256), 2 * nb * sizeof(double), s, a, nb, partials);
+  return 0;
+}
+
 }  // namespace sdns
 
 #define SDNS_INSTANTIATE(Rt)                                                                     \
   namespace sdns {                                                                               \
-  template int launch_p1<Rt>(int, const P1Args<Rt>&, hipStream_t);                               \
-  template int launch_p2<Rt>(int, const P2Args<Rt>&, hipStream_t);                               \
-  template int launch_p3<Rt>(int, const P3Args<Rt>&, hipStream_t);                               \
-  template int launch_p5<Rt>(int, int, const P5Args<Rt>&, hipStream_t, int*);                    \
-  template int launch_strided<Rt>(int, int, int, const StridedArgs<Rt>&, hipStream_t);           \
-  template int launch_z<Rt>(int, int, int, const ZArgs<Rt>&, hipStream_t);                       \
-  template int strided_cols<Rt>(int);                                                            \
+  template int launch_p1<Rt>(int, int, const P1Args<Rt>&, hipStream_t);                          \
+  template int launch_p2<Rt>(int, int, const P2Args<Rt>&, hipStream_t);                          \
+  template int launch_p3<Rt>(int, int, const P3Args<Rt>&, hipStream_t);                          \
+  template int launch_p5<Rt>(int, int, int, const P5Args<Rt>&, hipStream_t, int*);               \
+  template int launch_strided<Rt>(int, int, int, int, const StridedArgs<Rt>&, hipStream_t);      \
+  template int launch_z<Rt>(int, int, int, int, const ZArgs<Rt>&, hipStream_t);                  \
   template int launch_elementwise<Rt>(int, const EwArgs<Rt>&, hipStream_t);                      \
   template int launch_energy<Rt>(const EwArgs<Rt>&, double*, int, hipStream_t);                  \
+  template int launch_force_reduce<Rt>(const EwArgs<Rt>&, Rt, double*, int, hipStream_t);        \
+  template int launch_force_scale<Rt>(const EwArgs<Rt>&, Rt, const double*, double, hipStream_t); \
+  template int launch_spectrum<Rt>(const EwArgs<Rt>&, int, double*, int, hipStream_t);            \
   }

@@ -10,7 +10,9 @@
 //   W4 (y-forward out)     [3][M0][nkyo][nkzo]
 // nky/nkz are the kept input modes of the dealiased backward transform,
 // nkyo/nkzo the kept output modes of the dealiased forward transform, stored
-// compactly in fftfreq order (k >= 0 first, then the negative modes).
+// compactly in fftfreq order (k >= 0 first, then the negative modes).  The kz
+// dimension of the work buffers is padded to nkzp / nkzop (multiples of 16
+// elements) so that every row starts on a 128-byte line.
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -33,8 +35,8 @@ template <typename Rt> struct P1Args {  // x-inverse of u_hat + kx products
   int N0, N1, Nh;
   RowMap x;           // kept input modes of axis 0 (base = N0)
   int y_neg, y_pos;   // kept input modes of axis 1
-  int nkz;            // kept input kz count
-  int Q;              // nky * nkz columns
+  int nkz, nkzp;      // kept input kz count, padded kz stride of W1
+  int Q;              // nky * nkzp columns (= W1 row stride)
 };
 
 template <typename Rt> struct P2Args {  // y-inverse + curl assembly
@@ -45,8 +47,8 @@ template <typename Rt> struct P2Args {  // y-inverse + curl assembly
   const Rt* kz;       // wavenumbers of axis 2
   int N1, M0;
   RowMap y;           // kept input modes of axis 1 (base = nky)
-  int nkz;
-  int Q;              // M0 * nkz
+  int nkz, nkzp;
+  int Q;              // M0 * nkzp
 };
 
 template <typename Rt> struct P3Args {  // z C2R x6, cross product, z R2C x3
@@ -54,7 +56,7 @@ template <typename Rt> struct P3Args {  // z C2R x6, cross product, z R2C x3
   void* w3;           // [3][L][nkzo]
   const void* tw;     // forward twiddles of length n = M2/2
   const void* rtw;    // exp(-2 pi i p / M2), p in [0, n)
-  int nkz, nkzo;
+  int nkz, nkzp, nkzo, nkzop;
   int L;              // M0 * M1 lines
 };
 
@@ -65,7 +67,7 @@ template <typename Rt> struct StridedArgs {
   const void* in;
   void* out;
   const void* tw;
-  int Q, cdiv;
+  int Q, cdiv, cvalid;  // column q valid if q < Q and (q mod cdiv) < cvalid
   long in_hi, in_row, in_field;
   long out_hi, out_row, out_field;
   RowMap rin, rout;
@@ -101,7 +103,7 @@ template <typename Rt> struct P5Args {  // x-forward + mask + pressure + diffusi
   int N0, N1, Nh, N2;
   RowMap x;           // kept output modes of axis 0
   int y_neg, y_pos;   // kept output modes of axis 1
-  int nkzo;
+  int nkzo, nkzop;    // kept output kz count, padded kz stride of W4
   int mask_nyquist;
   int conv_only;      // P5_RHS: write the scaled convection term only
   Rt nu, scale, adt, bdt;

@@ -9,7 +9,8 @@ import numpy as np
 
 from ._abi import Plan, SdnsError
 
-__all__ = ["FourierSpace", "Array", "Function", "energy_fourier", "Wavenumbers", "NyquistMask", "WorkArrays"]
+__all__ = ["FourierSpace", "Array", "Function", "energy_fourier", "spectrum", "Wavenumbers", "NyquistMask",
+           "WorkArrays"]
 
 
 class Wavenumbers(list):
@@ -158,3 +159,27 @@ def energy_fourier(u_hat, T):
     out = torch.empty(2, dtype=torch.float64, device=plan.device)
     plan.energy(u_hat.contiguous(), 0.0, out)
     return float(out[0].item()) * 2.0
+
+
+def spectrum(u_hat, T, N=None):
+    """Shell-binned energy spectrum of demo/Isotropic.py:88-115 on the GPU.
+
+    Returns (Ek, bins) as NumPy arrays with the reference's definition: shells
+    (k - 1/2, k + 1/2] for k = 1..Nb-1 with Nb = int(sqrt(sum((N/2)^2)/3)),
+    Ek = (k_hi^3 - k_lo^3) * (4 pi/3) * sum(uiui) / count, Ek[Nb-1] = 0."""
+    plan = T.plan
+    N = np.array(plan.N if N is None else N)
+    Nb = int(np.sqrt(np.sum((N / 2) ** 2) / 3))
+    bins = np.array(range(0, Nb)) + 0.5
+    Ek = np.zeros(Nb)
+    if Nb < 2:
+        return Ek, bins
+    sums, counts = plan.spectrum_sums(u_hat.contiguous(), Nb - 1)
+    sums = sums.cpu().numpy() * (4.0 / 3.0 * np.pi)
+    counts = counts.cpu().numpy()
+    for i, k in enumerate(bins[1:]):
+        k0 = bins[i]
+        Ek[i] = (k ** 3 - k0 ** 3) * sums[i]
+        if counts[i] != 0:
+            Ek[i] = Ek[i] / counts[i]
+    return Ek, bins

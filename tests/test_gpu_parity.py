@@ -194,3 +194,29 @@ def test_256_cube_invariants(precision, rule):
     assert dv.abs().max().item() < tol * umax * 128
     h = ud.cpu().numpy()
     assert np.all(h[:, 128] == 0) and np.all(h[:, :, 128] == 0) and np.all(h[..., 128] == 0)
+
+
+@pytest.mark.parametrize("N,rule", [((128, 64, 256), "2/3-rule"), ((128, 128, 128), "3/2-rule"),
+                                    ((64, 32, 256), "3/2-rule")])
+def test_alternative_schedules_fp64(N, rule):
+    """Every pass run with its alternative line-FFT schedule (variant 1) keeps
+    FP64 parity with the oracle, for the RHS and the plain transforms."""
+    L = (TWO_PI, 3.0, 5.0)
+    nu = 0.01
+    p = plan_of(N, L, "double", rule)
+    for pid in range(6):
+        p.set_variant(pid, 1)
+    g = Grid(N, L, "double", rule)
+    u = field(N, L, seed=21)
+    ud = to_gpu(u, torch.complex128)
+    rhs = torch.empty_like(ud)
+    p.compute_rhs(ud, rhs, nu)
+    assert rel(rhs.cpu().numpy(), g.compute_rhs(u, nu)) < 1e-13
+    phys = torch.empty((3,) + N, dtype=torch.float64, device="cuda")
+    p.backward(ud, phys)
+    assert rel(phys.cpu().numpy(), g.backward(u)) < 1e-14
+    back = torch.empty_like(ud)
+    p.forward(phys, back)
+    assert rel(back.cpu().numpy(), u) < 1e-14
+    p.rk4_steps(ud, [0.01, 0.01], nu)
+    assert rel(ud.cpu().numpy(), g.rk4_step(g.rk4_step(u, 0.01, nu), 0.01, nu)) < 1e-12

@@ -53,7 +53,7 @@ def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, 
     res = {"variants": {}, "ms": {}}
     for pid, name in enumerate(PASSES):
         times = []
-        for var in (0, 1):
+        for var in ((0, 1, 2, 3) if pid == 2 else (0, 1)):
             plan.set_variant(pid, var)
             u = u0.clone()
             plan.rk4_steps(u, [1e-3], 1e-3)       # warm-up (first launch of this variant)

@@ -546,7 +546,9 @@ int sdns_plan_destroy(sdns_plan* p) {
 
 int sdns_plan_set_variant(sdns_plan* p, int pass, int variant) {
   if (!p) return fail(SDNS_ERR_INVALID, "null plan");
-  if (pass < 0 || pass > 5 || variant < 0 || variant > 1) return fail(SDNS_ERR_INVALID, "pass %d variant %d", pass, variant);
+  const int maxv = pass == 2 ? 3 : 1;
+  if (pass < 0 || pass > 5 || variant < 0 || variant > maxv)
+    return fail(SDNS_ERR_INVALID, "pass %d variant %d", pass, variant);
   p->var[pass] = variant;
   return 0;
 }

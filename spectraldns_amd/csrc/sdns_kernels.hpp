@@ -66,6 +66,18 @@ __device__ __forceinline__ int bin_row(int p, int M, RowMap m) {
 }
 template <typename C> __device__ __forceinline__ C czero() { return mk<C>(0, 0); }
 
+// Occupancy target (minimum waves per SIMD, the second __launch_bounds__
+// argument) from the number of complex values a thread keeps live: about
+// 1.6 registers per live 32-bit word plus fixed overhead, out of 512.  It caps
+// the register budget so that several blocks share a CU without spilling.
+constexpr int occ_for(int live_cx, int esz) {
+  const int regs = live_cx * (esz / 4) * 2 * 16 / 10 + 48;
+  const int w = 512 / regs;
+  return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+template <typename Rt> constexpr int esz_of() { return (int)sizeof(Rt); }
+constexpr int stage_k(int n, int P) { return (n + 1 + P - 1) / P; }
+
 // Column tile of this block.  Blocks are dispatched round-robin over the 8
 // XCDs; consecutive tiles go to blocks of the same XCD so that neighbouring
 // tiles, which share 128-byte lines of rows whose length is not a multiple
@@ -78,18 +90,26 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
 }
 inline int xcd_grid(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
+// Copy a twiddle table (cnt complex) into LDS; the caller synchronises.
+template <typename C>
+__device__ __forceinline__ void table_to_lds(C* dst, const C* __restrict__ src, int cnt) {
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ P1
 // blockIdx.y selects the component: y = 0 -> w (fields 2 and 3 = X(kx w)),
 // y = 1 -> v (fields 1 and 4 = X(kx v)), y = 2 -> u (field 0).
 template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p1(P1Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(2 * E, esz_of<Rt>()))) k_p1(P1Args<Rt> a) {
   using C = cx<Rt>;
   using Cfg = SCfg<Rt, M, E>;
   constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, +1>;
   __shared__ C lds[M * T];
+  __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
   if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;  // compact column iyc * nkzp + kz
   const int iyc = q / a.nkzp, kzi = q - iyc * a.nkzp;
@@ -99,13 +119,13 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p1(P1Args<Rt> a) 
   const int comp = 2 - (int)blockIdx.y;
   const long plane = (long)a.N1 * a.Nh;
   const C* __restrict__ U = (const C*)a.u + (long)comp * a.N0 * plane + (valid ? (long)iy * a.Nh + kzi : 0);
-  const C* __restrict__ tw = (const C*)a.tw;
   C v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int r = bin_row(t + P * e, M, a.x);
     v[e] = (valid && r >= 0) ? U[r * plane] : czero<C>();
   }
+  __syncthreads();
   C* __restrict__ out = (C*)a.w1 + q;
   const long orow = a.Q, ofield = (long)M * a.Q;
   C* line = lds + c;
@@ -134,14 +154,16 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p1(P1Args<Rt> a) 
 // x-inverse: wx = i(ky W - kz V), wy = i(kz U - X(kx W)), wz = i(X(kx V) - ky U).
 // blockIdx.y = 0: V, W -> wx, V, W ; blockIdx.y = 1: U, X(kx W), X(kx V) -> wy, wz, U.
 template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p2(P2Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p2(P2Args<Rt> a) {
   using C = cx<Rt>;
   using Cfg = SCfg<Rt, M, E>;
   constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, +1>;
   __shared__ C lds[M * T];
+  __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
   if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;  // column x * nkzp + kz
   const int x = q / a.nkzp, kzi = q - x * a.nkzp;
@@ -150,7 +172,6 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p2(P2Args<Rt> a) 
   const long ifield = (long)a.M0 * nky * a.nkzp, ofield = (long)a.M0 * M * a.nkzp;
   const C* __restrict__ in = (const C*)a.w1 + (valid ? (long)x * nky * a.nkzp + kzi : 0);
   C* __restrict__ out = (C*)a.w2 + (long)x * M * a.nkzp + kzi;
-  const C* __restrict__ tw = (const C*)a.tw;
   const int rs = a.nkzp;
   const Rt kzv = a.kz[valid ? kzi : 0];
   C* line = lds + c;
@@ -177,6 +198,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p2(P2Args<Rt> a) 
     C A[E], B[E], om[E];
     load(1, A);  // V
     load(2, B);  // W
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const Rt ky = kyv(e);
@@ -192,6 +214,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p2(P2Args<Rt> a) 
     C A[E], B[E];
     load(0, A);  // U
     load(3, B);  // X(kx W)
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e)
       B[e] = cmuli<1>(mk<C>(kzv * A[e].x - B[e].x, kzv * A[e].y - B[e].y));
@@ -288,20 +311,21 @@ struct LineStage {
 
 // ------------------------------------------------------------------ P3
 template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_p3(P3Args<Rt> a) {
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3(P3Args<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Fi = LineFFT<Rt, n, E, +1>;
   using Ff = LineFFT<Rt, n, E, -1>;
   __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C tw[n], rtw[n];
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
   const long li = valid ? line : 0;
   C* my = lds + l * (n + ZPAD);
-  const C* __restrict__ tw = (const C*)a.tw;
-  const C* __restrict__ rtw = (const C*)a.rtw;
   const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
   const long ifield = (long)a.L * a.nkzp;
   LineStage<Rt, n, P> st;
@@ -335,14 +359,109 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_p3(P3Args<Rt> a) 
   }
 }
 
+// P3, LDS-assisted scheme: the three velocity lines stay in registers, the
+// first two vorticity lines are parked in LDS (one slot per line each) and
+// only the last vorticity line is held in registers, so 4E complex values per
+// thread are live instead of 6E.  Three LDS slots per line serve as staging,
+// FFT exchange and parking space.
+constexpr int z3_lpb(int n, int P, int esz) {
+  int L = 256 / P;
+  if (L < 1) L = 1;
+  while (L > 1 && 3 * L * (n + ZPAD) * esz > 40960) L /= 2;
+  return L;
+}
+template <typename Rt, int n, int E_> struct ZBCfg {
+  static constexpr int E = E_;
+  static constexpr int P = n / E;
+  static constexpr int LPB = z3_lpb(n, P, (int)sizeof(cx<Rt>));
+  static constexpr int THREADS = LPB * P;
+};
+
+template <typename Rt, int n, int E>
+__global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3b(P3Args<Rt> a) {
+  using C = cx<Rt>;
+  using Cfg = ZBCfg<Rt, n, E>;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB, SL = n + ZPAD;
+  using Fi = LineFFT<Rt, n, E, +1>;
+  using Ff = LineFFT<Rt, n, E, -1>;
+  __shared__ C lds[3 * LPB * SL];
+  __shared__ C tw[n], rtw[n];
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
+  const int t = threadIdx.x % P, l = threadIdx.x / P;
+  const int line = blockIdx.x * LPB + l;
+  const bool valid = line < a.L;
+  const long li = valid ? line : 0;
+  C* s0 = lds + (3 * l) * SL;
+  C* s1 = s0 + SL;
+  C* s2 = s1 + SL;
+  const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
+  const long ifield = (long)a.L * a.nkzp;
+  LineStage<Rt, n, P> st;
+  st.load(in, a.nkz, valid, t);
+  C U[3][E], W[E];
+  // velocity lines -> registers (staging and exchange in slot 0)
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    st.store(s0, t);
+    __syncthreads();
+    c2r_pre_lds<Rt, n, E>(U[f], s0, rtw, t);
+    __syncthreads();
+    st.load(in + (f + 1) * ifield, a.nkz, valid, t);
+    Fi::run(U[f], t, s0, 1, tw);
+  }
+  // w0 -> slot 1, w1 -> slot 2 (each slot is its own staging and exchange area)
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    C* sl = f == 0 ? s1 : s2;
+    st.store(sl, t);
+    __syncthreads();
+    c2r_pre_lds<Rt, n, E>(W, sl, rtw, t);
+    __syncthreads();
+    st.load(in + (4 + f) * ifield, a.nkz, valid, t);
+    Fi::run(W, t, sl, 1, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) sl[t + P * e] = W[e];
+  }
+  // w2 -> registers (slot 0)
+  st.store(s0, t);
+  __syncthreads();
+  c2r_pre_lds<Rt, n, E>(W, s0, rtw, t);
+  __syncthreads();
+  Fi::run(W, t, s0, 1, tw);
+  __syncthreads();
+  // u x w on the packed points; the products overwrite the velocity registers
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    const C u0 = U[0][e], u1 = U[1][e], u2 = U[2][e];
+    const C w0 = s1[p], w1 = s2[p], w2 = W[e];
+    U[0][e] = mk<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+    U[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+    U[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+  }
+  __syncthreads();
+  C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
+  const long ofield = (long)a.L * a.nkzop;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    C* sl = i == 0 ? s0 : (i == 1 ? s1 : s2);
+    Ff::run(U[i], t, sl, 1, tw);
+    r2c_post<Rt, n, E>(U[i], sl, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+  }
+}
+
 // ------------------------------------------------------------------ plain z transforms
 template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_c2r(ZArgs<Rt> a) {
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_c2r(ZArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Fi = LineFFT<Rt, n, E, +1>;
   __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C tw[n], rtw[n];
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
@@ -354,9 +473,9 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_c2r(ZArgs<Rt> a
   st.store(my, t);
   __syncthreads();
   C z[E];
-  c2r_pre_lds<Rt, n, E>(z, my, (const C*)a.rtw, t);
+  c2r_pre_lds<Rt, n, E>(z, my, rtw, t);
   __syncthreads();
-  Fi::run(z, t, my, 1, (const C*)a.tw);
+  Fi::run(z, t, my, 1, tw);
   if (valid) {
     C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;  // real line viewed as n complex pairs
 #pragma unroll
@@ -365,12 +484,15 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_c2r(ZArgs<Rt> a
 }
 
 template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_r2c(ZArgs<Rt> a) {
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_r2c(ZArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
   using Ff = LineFFT<Rt, n, E, -1>;
   __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C tw[n], rtw[n];
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
@@ -380,21 +502,24 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS)) k_z_r2c(ZArgs<Rt> a
   C z[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) z[e] = valid ? in[t + P * e] : czero<C>();
-  Ff::run(z, t, my, 1, (const C*)a.tw);
+  __syncthreads();
+  Ff::run(z, t, my, 1, tw);
   C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;
-  r2c_post<Rt, n, E>(z, my, out, a.nk, valid, (const C*)a.rtw, t, a.scale);
+  r2c_post<Rt, n, E>(z, my, out, a.nk, valid, rtw, t, a.scale);
 }
 
 // ------------------------------------------------------------------ generic strided
 template <typename Rt, int M, int E, int DIR>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_strided(StridedArgs<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(E, esz_of<Rt>()))) k_strided(StridedArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = SCfg<Rt, M, E>;
   constexpr int P = Cfg::P, T = Cfg::T;
   using F = LineFFT<Rt, M, E, DIR>;
   __shared__ C lds[M * T];
+  __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
   if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;
   const int hi = q / a.cdiv, lo = q - hi * a.cdiv;
@@ -407,7 +532,8 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_strided(StridedAr
     const int r = bin_row(t + P * e, M, a.rin);
     v[e] = (valid && r >= 0) ? in[(long)r * a.in_row] : czero<C>();
   }
-  F::run(v, t, lds + c, T, (const C*)a.tw);
+  __syncthreads();
+  F::run(v, t, lds + c, T, tw);
   if (valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -419,15 +545,17 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_strided(StridedAr
 
 // ------------------------------------------------------------------ P5
 template <typename Rt, int M, int E, int MODE>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p5(P5Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p5(P5Args<Rt> a) {
   using C = cx<Rt>;
   using Cfg = SCfg<Rt, M, E>;
   constexpr int P = Cfg::P, T = Cfg::T, TH = Cfg::THREADS;
   using F = LineFFT<Rt, M, E, -1>;
   __shared__ C lds[M * T];
+  __shared__ C tw[M];
   const int ntiles = (a.Q + T - 1) / T;
   const int tile = xcd_tile(ntiles);
   if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;
   const bool valid = q < a.Q;
@@ -439,7 +567,6 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p5(P5Args<Rt> a) 
   const int kyc = ky >= 0 ? ky : nkyo + ky;
   const long wrow = (long)nkyo * a.nkzop, wfield = (long)M * wrow;
   const C* __restrict__ w4 = (const C*)a.w4 + (colconv ? (long)kyc * a.nkzop + kz : 0);
-  const C* __restrict__ tw = (const C*)a.tw;
   C X0[E], X1[E], X2[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -448,6 +575,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS)) k_p5(P5Args<Rt> a) 
     X1[e] = colconv ? w4[wfield + o] : czero<C>();
     X2[e] = colconv ? w4[2 * wfield + o] : czero<C>();
   }
+  __syncthreads();
   C* line = lds + c;
   F::run(X0, t, line, T, tw);
   F::run(X1, t, line, T, tw);

@@ -21,6 +21,10 @@ template <typename Rt, int n, int E> void p3_go(const P3Args<Rt>& a, hipStream_t
   using Cfg = ZCfg<Rt, n, E>;
   hipLaunchKernelGGL((k_p3<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
 }
+template <typename Rt, int n, int E> void p3b_go(const P3Args<Rt>& a, hipStream_t s) {
+  using Cfg = ZBCfg<Rt, n, E>;
+  hipLaunchKernelGGL((k_p3b<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
+}
 template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
@@ -74,11 +78,15 @@ template <typename Rt> int launch_p2(int M, int var, const P2Args<Rt>& a, hipStr
   }
 }
 
+// P3 variants: 0 and 1 = all six lines in registers with schedule 0 / 1;
+// 2 and 3 = the LDS-assisted scheme with schedule 0 / 1.
 template <typename Rt> int launch_p3(int n, int var, const P3Args<Rt>& a, hipStream_t s) {
   switch (n) {
 #define SDNS_X(m)                                                        \
   case m:                                                                \
-    if (var) p3_go<Rt, m, zline_E(m, 1)>(a, s);                          \
+    if (var == 3) p3b_go<Rt, m, zline_E(m, 1)>(a, s);                    \
+    else if (var == 2) p3b_go<Rt, m, zline_E(m, 0)>(a, s);               \
+    else if (var == 1) p3_go<Rt, m, zline_E(m, 1)>(a, s);                \
     else p3_go<Rt, m, zline_E(m, 0)>(a, s);                              \
     return 0;
     SDNS_HALF_SIZES(SDNS_X)

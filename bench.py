@@ -75,6 +75,44 @@ def gpu_run(N, precision, dealias, steps, warmup, nu=0.001, dt=0.001, seed=1234)
     return elapsed, passes, e
 
 
+def ensemble_run(steps, warmup, graph=True):
+    """The 96-box uneven ensemble (32 x 32^3, 32 x 48^3, 32 x 64^3, FP32,
+    2/3 rule): seconds for `steps` RK4 steps of every box, and the FP64
+    per-box (energy, dissipation) after them."""
+    import torch
+    from spectraldns_amd.ensemble import Ensemble
+    ens = Ensemble(precision="single")
+    ens.batch.use_graph(graph)
+    diag = torch.zeros(2 * ens.n, dtype=torch.float64, device="cuda")
+    ens.steps(warmup, diag)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.steps(steps, diag)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, ens, diag.cpu().numpy().reshape(-1, 2)
+
+
+def ensemble_main(args):
+    el, ens, diag = ensemble_run(args.steps, args.warmup, graph=not args.no_graph)
+    sps = args.steps / el
+    b_step = sum(pass_bytes((n, n, n), "2/3-rule", 8)[1] for n in ens.sizes)
+    out = {
+        "metric": "ensemble RK4 steps/s (every box advanced one step)",
+        "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * el / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Taylor-Green and seeded random-phase fields, nu log-uniform in [1e-3, 1e-2])",
+        "config": {"workload": "ensemble of 96 boxes: 32 x 32^3, 32 x 48^3, 32 x 64^3, 2/3 rule, RK4, FP32, "
+                               "dt=0.005, %s" % ("hipGraph replay" if not args.no_graph else "eager launches"),
+                   "boxes": ens.n, "points": ens.points()},
+        "box_steps_per_s": round(sps * ens.n, 1),
+        "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBps": round(b_step * sps / 1e9, 1),
+                          "frac": round(b_step * sps / 1e9 / HBM_PEAK_GBS, 4)},
+        "energy_range": [float(diag[:, 0].min()), float(diag[:, 0].max())],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
     from oracle.ns import Grid, _WORKERS
     from spectraldns_amd.isotropic import random_solenoidal_field
@@ -94,6 +132,9 @@ def main():
     ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble"),
+                    help="isotropic: the 256^3 box (default); ensemble: the 96-box uneven ensemble")
+    ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of graph replay")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first copy a buffer of this size 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -101,6 +142,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        ensemble_main(args)
+        return
     N = (args.N,) * 3
     if args.calibrate_copy:
         import torch

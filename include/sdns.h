@@ -98,9 +98,13 @@ int sdns_compute_rhs(sdns_plan* plan, const void* u_hat, void* rhs, double nu, c
  * is a device array of 2*nsteps doubles receiving after every step the
  * kinetic energy 0.5*sum w|u_hat|^2 and the dissipation nu*sum w|K x u_hat|^2
  * (FP64 reductions; w = 1 on kz = 0, N2/2 and 2 elsewhere).
- * u_lo: optional low-order words of u_hat for compensated single-precision
- * accumulation (NULL = plain).  p_hat, when not NULL, receives the pressure of
- * the last RHS evaluation, as the reference leaves P_hat after a step. */
+ * u_lo (single precision only; ignored in double): low-order part of the
+ * state for compensated accumulation.  The state is then u_hat + u_lo, the
+ * RK4 sum u2 is carried as an exact (hi, lo) pair with TwoSum updates, and
+ * after each step u_hat = fl(u_hat + u_lo) and u_lo holds the remainder;
+ * start it at zero.  NULL = plain single-precision arithmetic.  p_hat, when
+ * not NULL, receives the pressure of the last RHS evaluation, as the
+ * reference leaves P_hat after a step. */
 int sdns_rk4_steps(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_seq, int nsteps, double nu,
                    double* diag, void* p_hat);
 
@@ -129,6 +133,44 @@ int sdns_spectrum(sdns_plan* plan, const void* u_hat, int nbins, double* out);
  * RK stage, 5: other launches) and clears the record. */
 int sdns_timing_enable(sdns_plan* plan, int enable);
 int sdns_timing_read(sdns_plan* plan, double ms[6], int count[6]);
+
+/* ------------------------------------------------------------------------
+ * Batched ensembles of independent boxes.  No counterpart in the reference,
+ * which advances one box per process (spectralDNS/__init__.py:69-123); each
+ * box of a batch takes exactly the RK4 step of sdns_rk4_steps
+ * (integrators.py:149-159 over ComputeRHS, NS.py:219-261) with its own N, L,
+ * nu and dt.  One step of the whole ensemble is 20 pass launches, each
+ * covering every box, plus one reduction, captured once as a hipGraph on the
+ * batch's own stream and replayed per step.
+ *
+ * sdns_batch_create: nbox boxes, N and L of box i at N[3i..3i+2], L[3i..3i+2];
+ * precision, dealias (2/3 or 3/2 rule), mask_nyquist and dealias_mode as in
+ * sdns_plan_create and shared by all boxes.  Every dealiased line length
+ * (3N/2 under the 3/2 rule) must have a batched kernel: x and y in
+ * {16, 24, 32, 48, 64, 96, 128}, z/2 in {8, 12, 16, 24, 32, 48, 64}; otherwise
+ * SDNS_ERR_UNSUPPORTED.  hip_stream is the caller's stream (NULL = null
+ * stream): sdns_batch_rk4_steps orders its work after what the caller has
+ * enqueued there and makes that stream wait for its completion. */
+typedef struct sdns_batch sdns_batch;
+int sdns_batch_create(sdns_batch** batch, int nbox, const int32_t* N, const double* L, int precision,
+                      int dealias, int mask_nyquist, int dealias_mode, void* hip_stream);
+int sdns_batch_destroy(sdns_batch* batch);
+/* u_hat[i]: device pointer of box i's state (3, N0, N1, N2/2+1), caller-owned;
+ * advanced in place.  The batch keeps the pointers until they are set again. */
+int sdns_batch_set_state(sdns_batch* batch, void* const* u_hat);
+/* Viscosity and time step of every box (host arrays of nbox doubles; nu and
+ * dt are rounded to the working precision as config.py:123-127 does). */
+int sdns_batch_set_params(sdns_batch* batch, const double* nu, const double* dt);
+/* 1 (default): replay the captured step graph; 0: enqueue the launches. */
+int sdns_batch_use_graph(sdns_batch* batch, int enable);
+int sdns_batch_set_stream(sdns_batch* batch, void* hip_stream);
+/* nsteps RK4 steps of every box.  diag, when not NULL, is a device array of
+ * 2*nbox doubles receiving (kinetic energy, dissipation) of every box after
+ * the last step, in FP64, with the definitions of sdns_rk4_steps. */
+int sdns_batch_rk4_steps(sdns_batch* batch, int nsteps, double* diag);
+/* The single-box plan of box i (owned by the batch; for diagnostics such as
+ * sdns_energy or sdns_spectrum on that box). */
+int sdns_batch_box_plan(sdns_batch* batch, int i, sdns_plan** plan);
 
 /* Human-readable message of the last failing call on this thread. */
 const char* sdns_last_error(void);

@@ -42,6 +42,16 @@ _SIGS = {
     "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "sdns_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, _P]),
+    "sdns_batch_destroy": (ctypes.c_int, [_P]),
+    "sdns_batch_set_state": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "sdns_batch_set_params": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "sdns_batch_use_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sdns_batch_set_stream": (ctypes.c_int, [_P, _P]),
+    "sdns_batch_rk4_steps": (ctypes.c_int, [_P, ctypes.c_int, _P]),
+    "sdns_batch_box_plan": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P)]),
     "sdns_last_error": (ctypes.c_char_p, []),
 }
 
@@ -294,3 +304,76 @@ class Plan:
         cnt = (ctypes.c_int * 6)()
         _check(self._lib.sdns_timing_read(self._h, ms, cnt))
         return {n: (ms[i], cnt[i]) for i, n in enumerate(self.PASS_NAMES)}
+
+
+class Batch:
+    """Ensemble of independent boxes (own N, L, nu, dt) advanced together:
+    every pass of the RK4 step is one launch over all boxes and the step is
+    replayed from a captured hipGraph (sdns_batch_* in include/sdns.h)."""
+
+    def __init__(self, boxes, precision="single", dealias="2/3-rule", mask_nyquist=True, dealias_mode=0):
+        import torch
+        lib = load()
+        if not torch.cuda.is_available():
+            raise SdnsError("no GPU visible: the HIP path has no CPU fallback")
+        self.boxes = [(tuple(int(n) for n in N), tuple(float(x) for x in L)) for N, L in boxes]
+        if not self.boxes:
+            raise SdnsError("an ensemble needs at least one box")
+        self.nbox = len(self.boxes)
+        self.precision = precision
+        self.complex_dtype = torch.complex64 if precision == "single" else torch.complex128
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        Na = (ctypes.c_int32 * (3 * self.nbox))(*[n for N, _ in self.boxes for n in N])
+        La = (ctypes.c_double * (3 * self.nbox))(*[x for _, L in self.boxes for x in L])
+        h = _P()
+        _check(lib.sdns_batch_create(ctypes.byref(h), self.nbox, Na, La,
+                                     SDNS_SINGLE if precision == "single" else SDNS_DOUBLE,
+                                     DEALIAS[dealias], int(bool(mask_nyquist)), int(dealias_mode),
+                                     _P(torch.cuda.current_stream().cuda_stream)))
+        self._h = h
+        self._lib = lib
+        self._state = None
+        self.spectral_shapes = [(3, N[0], N[1], N[2] // 2 + 1) for N, _ in self.boxes]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.sdns_batch_destroy(h)
+            self._h = None
+
+    def set_state(self, u_hats):
+        """Box i advances u_hats[i] in place (contiguous (3, N0, N1, N2/2+1) GPU tensors)."""
+        u_hats = list(u_hats)
+        if len(u_hats) != self.nbox:
+            raise SdnsError("%d states for %d boxes" % (len(u_hats), self.nbox))
+        for i, (t, shp) in enumerate(zip(u_hats, self.spectral_shapes)):
+            if t.device.type != "cuda" or t.dtype != self.complex_dtype or not t.is_contiguous() \
+                    or tuple(t.shape) != shp:
+                raise SdnsError("box %d: state must be a contiguous %s GPU tensor of shape %s"
+                                % (i, self.complex_dtype, shp))
+        arr = (_P * self.nbox)(*[t.data_ptr() for t in u_hats])
+        _check(self._lib.sdns_batch_set_state(self._h, arr))
+        self._state = u_hats  # keep the tensors alive while the batch holds their pointers
+
+    def set_params(self, nu, dt):
+        nu = [float(x) for x in nu]
+        dt = [float(x) for x in dt]
+        if len(nu) != self.nbox or len(dt) != self.nbox:
+            raise SdnsError("nu and dt need one value per box")
+        _check(self._lib.sdns_batch_set_params(self._h, (ctypes.c_double * self.nbox)(*nu),
+                                               (ctypes.c_double * self.nbox)(*dt)))
+
+    def use_graph(self, on=True):
+        _check(self._lib.sdns_batch_use_graph(self._h, int(bool(on))))
+
+    def rk4_steps(self, nsteps, diag=None):
+        """nsteps RK4 steps of every box; diag (float64 GPU tensor of 2*nbox
+        entries) receives (energy, dissipation) per box after the last step."""
+        import torch
+        if self._state is None:
+            raise SdnsError("set_state first")
+        if diag is not None and (diag.dtype != torch.float64 or diag.device.type != "cuda"
+                                 or diag.numel() < 2 * self.nbox or not diag.is_contiguous()):
+            raise SdnsError("diag must be a contiguous float64 GPU tensor of >= 2*nbox entries")
+        _check(self._lib.sdns_batch_set_stream(self._h, _P(torch.cuda.current_stream().cuda_stream)))
+        _check(self._lib.sdns_batch_rk4_steps(self._h, int(nsteps), _ptr(diag) if diag is not None else None))

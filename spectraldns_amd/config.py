@@ -140,6 +140,9 @@ parser.add_argument("--dealias", default="2/3-rule", choices=("2/3-rule", "3/2-r
                     help="Choose dealiasing method")
 parser.add_argument("--dealias_mode", default=0, type=int, choices=(0, 1),
                     help="2/3-rule variant: 0 truncate inputs |k| < N/3, 1 output mask |k| < 2/3(N/2+1)")
+parser.add_argument("--compensated", dest="compensated", action="store_true",
+                    help="single precision: carry u_hat as a compensated (hi, lo) pair in RK4")
+parser.set_defaults(compensated=False)
 parser.add_argument("--decomposition", default="slab", choices=("slab", "pencil"),
                     help="Kept for compatibility (one GPU holds the whole box)")
 parser.add_argument("--ntol", default=7, type=int, help="Tolerance - number of accurate digits")

@@ -96,7 +96,7 @@ struct sdns_plan {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   bool ws_owned = false;
-  char *bufA = nullptr, *bufB = nullptr, *stB = nullptr, *stC = nullptr;
+  char *bufA = nullptr, *bufB = nullptr, *stB = nullptr, *stC = nullptr, *stClo = nullptr;
   double* part = nullptr;
   int max_blocks = 0;
   // per-pass event timing
@@ -174,7 +174,11 @@ template <typename Rt> int build_tables(sdns_plan* p) {
   if ((rc = upload_twiddles<Rt>(p, p->N[1], false))) return rc;
   if ((rc = upload_twiddles<Rt>(p, p->nz_plain, false))) return rc;
   if ((rc = upload_twiddles<Rt>(p, p->N[2], true))) return rc;
-  return upload_k<Rt>(p);
+  if ((rc = upload_k<Rt>(p))) return rc;
+  // hipMemcpy from pageable memory may return before its DMA lands; the plan
+  // may run on any stream, so wait for the tables here
+  HIP_TRY(hipDeviceSynchronize());
+  return 0;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -194,7 +198,7 @@ void compute_sizes(sdns_plan* p) {
 }
 
 int assign_workspace(sdns_plan* p, void* base, size_t bytes) {
-  const size_t need = p->szA + p->szB + 2 * p->szS + p->szPart;
+  const size_t need = p->szA + p->szB + 3 * p->szS + p->szPart;
   if (bytes < need) return fail(SDNS_ERR_INVALID, "workspace of %zu bytes, need %zu", bytes, need);
   if (((uintptr_t)base) & 255) return fail(SDNS_ERR_INVALID, "workspace must be 256-byte aligned");
   char* c = (char*)base;
@@ -202,6 +206,7 @@ int assign_workspace(sdns_plan* p, void* base, size_t bytes) {
   p->bufB = c; c += p->szB;
   p->stB = c; c += p->szS;
   p->stC = c; c += p->szS;
+  p->stClo = c; c += p->szS;
   p->part = (double*)c;
   p->ws = base;
   p->ws_bytes = bytes;
@@ -210,7 +215,7 @@ int assign_workspace(sdns_plan* p, void* base, size_t bytes) {
 
 int ensure_workspace(sdns_plan* p) {
   if (p->ws) return 0;
-  const size_t need = p->szA + p->szB + 2 * p->szS + p->szPart;
+  const size_t need = p->szA + p->szB + 3 * p->szS + p->szPart;
   void* d = nullptr;
   if (hipMalloc(&d, need) != hipSuccess) return fail(SDNS_ERR_NOMEM, "hipMalloc of %zu bytes failed", need);
   p->ws_owned = true;
@@ -286,47 +291,65 @@ template <typename Rt> int plain_forward(sdns_plan* p, const void* up, void* uh,
   return 0;
 }
 
-// P1..P4 of one RHS evaluation from the state u (device) into W4 (= bufB).
+// Argument blocks of P1..P4 of one RHS evaluation from the state u (device)
+// into W4 (= bufB); shared by the single-box path and the batched ensembles.
+template <typename Rt> P1Args<Rt> p1_args(const sdns_plan* p, const void* u) {
+  P1Args<Rt> a{};
+  a.u = u; a.w1 = p->bufA; a.tw = p->tw.at(p->M[0]); a.kx = (const Rt*)p->ktab[0];
+  a.N0 = p->N[0]; a.N1 = p->N[1]; a.Nh = p->Nh;
+  a.x = rmap(p->in[0][0], p->in[0][1], p->N[0]);
+  a.y_neg = p->in[1][0]; a.y_pos = p->in[1][1]; a.nkz = p->nkz; a.nkzp = p->nkzp; a.Q = p->nky * p->nkzp;
+  return a;
+}
+template <typename Rt> P2Args<Rt> p2_args(const sdns_plan* p) {
+  P2Args<Rt> a{};
+  a.w1 = p->bufA; a.w2 = p->bufB; a.tw = p->tw.at(p->M[1]); a.ky = (const Rt*)p->ktab[1];
+  a.kz = (const Rt*)p->ktab[2];
+  a.N1 = p->N[1]; a.M0 = p->M[0]; a.y = rmap(p->in[1][0], p->in[1][1], p->nky); a.nkz = p->nkz; a.nkzp = p->nkzp;
+  a.Q = p->M[0] * p->nkzp;
+  return a;
+}
+template <typename Rt> P3Args<Rt> p3_args(const sdns_plan* p) {
+  P3Args<Rt> a{};
+  a.w2 = p->bufB; a.w3 = p->bufA; a.tw = p->tw.at(p->n2h); a.rtw = p->rtw.at(p->M[2]);
+  a.nkz = p->nkz; a.nkzp = p->nkzp; a.nkzo = p->nkzo; a.nkzop = p->nkzop; a.L = p->M[0] * p->M[1];
+  return a;
+}
+template <typename Rt> StridedArgs<Rt> p4_args(const sdns_plan* p) {
+  const int M0 = p->M[0], M1 = p->M[1];
+  StridedArgs<Rt> a{};
+  a.in = p->bufA; a.out = p->bufB; a.tw = p->tw.at(M1);
+  a.Q = M0 * p->nkzop; a.cdiv = p->nkzop; a.cvalid = p->nkzo;
+  a.in_hi = (long)M1 * p->nkzop; a.in_row = p->nkzop; a.in_field = (long)M0 * M1 * p->nkzop;
+  a.out_hi = (long)p->nkyo * p->nkzop; a.out_row = p->nkzop; a.out_field = (long)M0 * p->nkyo * p->nkzop;
+  a.rin = rmap(M1 / 2, M1 - M1 / 2 - 1, M1);
+  a.rout = rmap(p->out[1][0], p->out[1][1], p->nkyo);
+  a.scale = Rt(1);
+  return a;
+}
+
 template <typename Rt> int conv_passes(sdns_plan* p, const void* u) {
   const int M0 = p->M[0], M1 = p->M[1];
-  P1Args<Rt> a1{};
-  a1.u = u; a1.w1 = p->bufA; a1.tw = p->tw[M0]; a1.kx = (const Rt*)p->ktab[0];
-  a1.N0 = p->N[0]; a1.N1 = p->N[1]; a1.Nh = p->Nh;
-  a1.x = rmap(p->in[0][0], p->in[0][1], p->N[0]);
-  a1.y_neg = p->in[1][0]; a1.y_pos = p->in[1][1]; a1.nkz = p->nkz; a1.nkzp = p->nkzp; a1.Q = p->nky * p->nkzp;
   tstart(p, 0);
-  if (launch_p1<Rt>(M0, p->var[0], a1, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
+  if (launch_p1<Rt>(M0, p->var[0], p1_args<Rt>(p, u), p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
   tstop(p);
-  P2Args<Rt> a2{};
-  a2.w1 = p->bufA; a2.w2 = p->bufB; a2.tw = p->tw[M1]; a2.ky = (const Rt*)p->ktab[1]; a2.kz = (const Rt*)p->ktab[2];
-  a2.N1 = p->N[1]; a2.M0 = M0; a2.y = rmap(p->in[1][0], p->in[1][1], p->nky); a2.nkz = p->nkz; a2.nkzp = p->nkzp;
-  a2.Q = M0 * p->nkzp;
   tstart(p, 1);
-  if (launch_p2<Rt>(M1, p->var[1], a2, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  if (launch_p2<Rt>(M1, p->var[1], p2_args<Rt>(p), p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
   tstop(p);
-  P3Args<Rt> a3{};
-  a3.w2 = p->bufB; a3.w3 = p->bufA; a3.tw = p->tw[p->n2h]; a3.rtw = p->rtw[p->M[2]];
-  a3.nkz = p->nkz; a3.nkzp = p->nkzp; a3.nkzo = p->nkzo; a3.nkzop = p->nkzop; a3.L = M0 * M1;
   tstart(p, 2);
-  if (launch_p3<Rt>(p->n2h, p->var[2], a3, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", p->M[2]);
+  if (launch_p3<Rt>(p->n2h, p->var[2], p3_args<Rt>(p), p->stream))
+    return fail(SDNS_ERR_UNSUPPORTED, "z length %d", p->M[2]);
   tstop(p);
-  StridedArgs<Rt> a4{};
-  a4.in = p->bufA; a4.out = p->bufB; a4.tw = p->tw[M1];
-  a4.Q = M0 * p->nkzop; a4.cdiv = p->nkzop; a4.cvalid = p->nkzo;
-  a4.in_hi = (long)M1 * p->nkzop; a4.in_row = p->nkzop; a4.in_field = (long)M0 * M1 * p->nkzop;
-  a4.out_hi = (long)p->nkyo * p->nkzop; a4.out_row = p->nkzop; a4.out_field = (long)M0 * p->nkyo * p->nkzop;
-  a4.rin = rmap(M1 / 2, M1 - M1 / 2 - 1, M1);
-  a4.rout = rmap(p->out[1][0], p->out[1][1], p->nkyo);
-  a4.scale = Rt(1);
   tstart(p, 3);
-  if (launch_strided<Rt>(M1, p->var[3], -1, 3, a4, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  if (launch_strided<Rt>(M1, p->var[3], -1, 3, p4_args<Rt>(p), p->stream))
+    return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
   tstop(p);
   return 0;
 }
 
-template <typename Rt> P5Args<Rt> p5_args(sdns_plan* p, double nu) {
+template <typename Rt> P5Args<Rt> p5_args(const sdns_plan* p, double nu) {
   P5Args<Rt> a{};
-  a.w4 = p->bufB; a.tw = p->tw[p->M[0]];
+  a.w4 = p->bufB; a.tw = p->tw.at(p->M[0]);
   a.kx = (const Rt*)p->ktab[0]; a.ky = (const Rt*)p->ktab[1]; a.kz = (const Rt*)p->ktab[2];
   a.N0 = p->N[0]; a.N1 = p->N[1]; a.Nh = p->Nh; a.N2 = p->N[2];
   a.x = rmap(p->out[0][0], p->out[0][1], p->N[0]);
@@ -360,8 +383,8 @@ template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs) {
   return 0;
 }
 
-template <typename Rt> int rk4_steps(sdns_plan* p, void* u, const double* dts, int nsteps, double nu, double* diag,
-                                      void* ph) {
+template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const double* dts, int nsteps, double nu,
+                                      double* diag, void* ph) {
   // a, b and dt in the working precision (integrators.py:185-186, config.py:123-127)
   const Rt a[4] = {Rt(1. / 6.), Rt(1. / 3.), Rt(1. / 3.), Rt(1. / 6.)};
   const Rt b[3] = {Rt(0.5), Rt(0.5), Rt(1.)};
@@ -373,6 +396,7 @@ template <typename Rt> int rk4_steps(sdns_plan* p, void* u, const double* dts, i
       if ((rc = conv_passes<Rt>(p, rk == 0 ? u : (const void*)p->stB))) return rc;
       P5Args<Rt> pa = p5_args<Rt>(p, nu);
       pa.A = u; pa.B = p->stB; pa.Cacc = p->stC;
+      if (ulo) { pa.Alo = ulo; pa.Clo = p->stClo; }
       pa.adt = a[rk] * dt;
       pa.bdt = rk < 3 ? b[rk] * dt : Rt(0);
       const int mode = rk == 0 ? P5_RK0 : (rk < 3 ? P5_RK12 : P5_RK3);
@@ -476,7 +500,237 @@ int launch_sum_rows(const double* part, int nblk, int width, double* out, hipStr
   hipLaunchKernelGGL(k_sum_rows, dim3((width + 255) / 256), dim3(256), 0, s, part, nblk, width, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+// Per-box (energy, dissipation) of a batched ensemble: block b sums box b's
+// P5 partials part[off[b] .. off[b] + 2*cnt[b]) in a fixed order.
+__global__ void __launch_bounds__(256) k_batch_sum(const double* __restrict__ part, const int* __restrict__ off,
+                                                   const int* __restrict__ cnt, const double* __restrict__ nu,
+                                                   double* __restrict__ out) {
+  __shared__ double s[2][256];
+  const int b = blockIdx.x;
+  const double* pb = part + off[b];
+  double a0 = 0, a1 = 0;
+  for (int i = threadIdx.x; i < cnt[b]; i += 256) { a0 += pb[2 * i]; a1 += pb[2 * i + 1]; }
+  s[0][threadIdx.x] = a0;
+  s[1][threadIdx.x] = a1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { s[0][threadIdx.x] += s[0][threadIdx.x + w]; s[1][threadIdx.x] += s[1][threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[2 * b] = 0.5 * s[0][0]; out[2 * b + 1] = nu[b] * s[1][0]; }
+}
 }  // namespace sdns
+
+// ---------------------------------------------------------------- batched ensembles
+// Every box keeps its own plan (tables, work buffers, RK stage buffers); the
+// batch adds, in one device allocation, the per-box argument blocks of each
+// pass, the line length of each box per pass and the block work lists.  One
+// RK4 step is 20 batched pass launches plus one reduction, enqueued on the
+// batch's own stream and captured once as a hipGraph.
+struct sdns_batch {
+  int nbox = 0, precision = 0;
+  std::vector<sdns_plan*> box;
+  hipStream_t user = nullptr, own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  char* dmem = nullptr;
+  // device arrays inside dmem
+  void* d_p1[2] = {nullptr, nullptr};  // P1 args: stage 0 reads u_hat, stages 1-3 the stage buffer
+  void *d_p2 = nullptr, *d_p3 = nullptr, *d_p4 = nullptr;
+  void* d_p5[4] = {nullptr, nullptr, nullptr, nullptr};  // P5 args of RK stages 0..3
+  int* d_len[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  BatchWork* d_work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nwork[5] = {0, 0, 0, 0, 0}, lmax[5] = {0, 0, 0, 0, 0};
+  double *d_part = nullptr, *d_diag = nullptr, *d_nu = nullptr;
+  int *d_poff = nullptr, *d_pcnt = nullptr;
+  size_t args_bytes = 0;  // bytes of one per-box argument array (largest struct * nbox)
+  std::vector<int> poff, pcnt;
+  std::vector<void*> u;
+  std::vector<double> nu, dt;
+  bool have_state = false, have_params = false, dirty = true, use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Work list of a batched pass: items (box, tile) in box order, split into 8
+// contiguous chunks of about equal work (weight per tile of each box), chunk
+// x executed by the blocks i with i % 8 == x (the XCD that dispatches them),
+// so that the tiles of one box share an L2.  Padding items have box = -1.
+std::vector<BatchWork> batch_work(const std::vector<int>& ntile, const std::vector<int>& weight) {
+  std::vector<BatchWork> items;
+  long total = 0;
+  for (size_t b = 0; b < ntile.size(); ++b) {
+    for (int t = 0; t < ntile[b]; ++t) items.push_back({(int)b, t});
+    total += (long)ntile[b] * weight[b];
+  }
+  std::vector<std::vector<BatchWork>> chunk(8);
+  long acc = 0;
+  for (const BatchWork& w : items) {
+    int x = (int)((acc * 8) / std::max(total, 1L));
+    if (x > 7) x = 7;
+    chunk[x].push_back(w);
+    acc += weight[w.box];
+  }
+  size_t per = 0;
+  for (auto& c : chunk) per = std::max(per, c.size());
+  std::vector<BatchWork> out(8 * std::max(per, (size_t)1), BatchWork{-1, 0});
+  for (int x = 0; x < 8; ++x)
+    for (size_t j = 0; j < chunk[x].size(); ++j) out[8 * j + x] = chunk[x][j];
+  return out;
+}
+
+bool batch_strided_ok(int M) { return batch_E(M) > 0; }
+bool batch_half_ok(int n) { return batch_zE(n) > 0; }
+
+template <typename Rt> int batch_layout(sdns_batch* b) {
+  const int nb = b->nbox;
+  std::vector<int> len[5], nt[5], wt[5];
+  for (int i = 0; i < nb; ++i) {
+    const sdns_plan* p = b->box[i];
+    const int M0 = p->M[0], M1 = p->M[1], n = p->n2h;
+    const int l[5] = {M0, M1, n, M1, M0};
+    const int T1 = batch_T<Rt>(M0), T2 = batch_T<Rt>(M1), LPB = batch_LPB<Rt>(n);
+    const int q[5] = {p->nky * p->nkzp, M0 * p->nkzp, M0 * M1, M0 * p->nkzop, p->N[1] * p->Nh};
+    const int per[5] = {T1, T2, LPB, T2, T1};
+    for (int k = 0; k < 5; ++k) {
+      len[k].push_back(l[k]);
+      nt[k].push_back((q[k] + per[k] - 1) / per[k]);
+      wt[k].push_back(k == 2 ? batch_zE(l[k]) : batch_E(l[k]));
+      b->lmax[k] = std::max(b->lmax[k], l[k]);
+    }
+  }
+  std::vector<BatchWork> work[5];
+  size_t nitems = 0;
+  for (int k = 0; k < 5; ++k) {
+    work[k] = batch_work(nt[k], wt[k]);
+    b->nwork[k] = (int)work[k].size() / 8 * 8;
+    nitems += work[k].size();
+  }
+  b->poff.assign(nb, 0);
+  b->pcnt = nt[4];
+  int part_total = 0;
+  for (int i = 0; i < nb; ++i) { b->poff[i] = part_total; part_total += 2 * nt[4][i]; }
+  // one allocation: 9 argument arrays, lengths, work lists, partials, diag, nu, offsets
+  size_t argsz = std::max(std::max(sizeof(P1Args<Rt>), sizeof(P2Args<Rt>)),
+                          std::max(std::max(sizeof(P3Args<Rt>), sizeof(StridedArgs<Rt>)), sizeof(P5Args<Rt>)));
+  b->args_bytes = align256(argsz * nb);
+  const size_t bytes = 9 * b->args_bytes + align256(5 * nb * sizeof(int)) + align256(nitems * sizeof(BatchWork)) +
+                       align256(part_total * sizeof(double)) + 2 * align256(2 * nb * sizeof(double)) +
+                       2 * align256(nb * sizeof(int));
+  if (hipMalloc((void**)&b->dmem, bytes) != hipSuccess) return fail(SDNS_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+  char* c = b->dmem;
+  void** slots[9] = {&b->d_p1[0], &b->d_p1[1], &b->d_p2, &b->d_p3, &b->d_p4,
+                     &b->d_p5[0], &b->d_p5[1], &b->d_p5[2], &b->d_p5[3]};
+  for (auto sl : slots) { *sl = c; c += b->args_bytes; }
+  int* lens = (int*)c;
+  c += align256(5 * nb * sizeof(int));
+  BatchWork* wl = (BatchWork*)c;
+  c += align256(nitems * sizeof(BatchWork));
+  b->d_part = (double*)c; c += align256(part_total * sizeof(double));
+  b->d_diag = (double*)c; c += align256(2 * nb * sizeof(double));
+  b->d_nu = (double*)c; c += align256(2 * nb * sizeof(double));
+  b->d_poff = (int*)c; c += align256(nb * sizeof(int));
+  b->d_pcnt = (int*)c;
+  std::vector<int> hl;
+  for (int k = 0; k < 5; ++k) hl.insert(hl.end(), len[k].begin(), len[k].end());
+  HIP_TRY(hipMemcpyAsync(lens, hl.data(), hl.size() * sizeof(int), hipMemcpyHostToDevice, b->own));
+  size_t o = 0;
+  for (int k = 0; k < 5; ++k) {
+    b->d_len[k] = lens + k * nb;
+    b->d_work[k] = wl + o;
+    HIP_TRY(hipMemcpyAsync(wl + o, work[k].data(), work[k].size() * sizeof(BatchWork), hipMemcpyHostToDevice,
+                           b->own));
+    o += work[k].size();
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_poff, b->poff.data(), nb * sizeof(int), hipMemcpyHostToDevice, b->own));
+  HIP_TRY(hipMemcpyAsync(b->d_pcnt, b->pcnt.data(), nb * sizeof(int), hipMemcpyHostToDevice, b->own));
+  HIP_TRY(hipStreamSynchronize(b->own));
+  return 0;
+}
+
+// Per-box argument arrays for the current state pointers, nu and dt.
+template <typename Rt> int batch_upload(sdns_batch* b) {
+  const int nb = b->nbox;
+  const Rt a[4] = {Rt(1. / 6.), Rt(1. / 3.), Rt(1. / 3.), Rt(1. / 6.)};
+  const Rt bb[3] = {Rt(0.5), Rt(0.5), Rt(1.)};
+  std::vector<P1Args<Rt>> h1[2];
+  std::vector<P2Args<Rt>> h2;
+  std::vector<P3Args<Rt>> h3;
+  std::vector<StridedArgs<Rt>> h4;
+  std::vector<P5Args<Rt>> h5[4];
+  for (int i = 0; i < nb; ++i) {
+    const sdns_plan* p = b->box[i];
+    h1[0].push_back(p1_args<Rt>(p, b->u[i]));
+    h1[1].push_back(p1_args<Rt>(p, p->stB));
+    h2.push_back(p2_args<Rt>(p));
+    h3.push_back(p3_args<Rt>(p));
+    h4.push_back(p4_args<Rt>(p));
+    const Rt dt = (Rt)b->dt[i];
+    for (int rk = 0; rk < 4; ++rk) {  // stages 1 and 2 share a but not b
+      P5Args<Rt> pa = p5_args<Rt>(p, b->nu[i]);
+      pa.A = b->u[i]; pa.B = p->stB; pa.Cacc = p->stC;
+      pa.adt = a[rk] * dt;
+      pa.bdt = rk < 3 ? bb[rk] * dt : Rt(0);
+      pa.partials = rk == 3 ? b->d_part + b->poff[i] : nullptr;
+      h5[rk].push_back(pa);
+    }
+  }
+  // copies on the batch stream, which is then drained: the host vectors stay
+  // alive until the data are on the device and no replay can see a partial update
+  hipStream_t s = b->own;
+  HIP_TRY(hipMemcpyAsync(b->d_p1[0], h1[0].data(), nb * sizeof(P1Args<Rt>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_p1[1], h1[1].data(), nb * sizeof(P1Args<Rt>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_p2, h2.data(), nb * sizeof(P2Args<Rt>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_p3, h3.data(), nb * sizeof(P3Args<Rt>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_p4, h4.data(), nb * sizeof(StridedArgs<Rt>), hipMemcpyHostToDevice, s));
+  for (int m = 0; m < 4; ++m)
+    HIP_TRY(hipMemcpyAsync(b->d_p5[m], h5[m].data(), nb * sizeof(P5Args<Rt>), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_nu, b->nu.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// One RK4 step of every box on stream s.
+template <typename Rt> int batch_step(sdns_batch* b, hipStream_t s) {
+  for (int rk = 0; rk < 4; ++rk) {
+    launch_batch_p1<Rt>(b->lmax[0], (const P1Args<Rt>*)b->d_p1[rk == 0 ? 0 : 1], b->d_len[0], b->d_work[0],
+                        b->nwork[0], s);
+    launch_batch_p2<Rt>(b->lmax[1], (const P2Args<Rt>*)b->d_p2, b->d_len[1], b->d_work[1], b->nwork[1], s);
+    launch_batch_p3<Rt>(b->lmax[2], (const P3Args<Rt>*)b->d_p3, b->d_len[2], b->d_work[2], b->nwork[2], s);
+    launch_batch_fwd<Rt>(b->lmax[3], (const StridedArgs<Rt>*)b->d_p4, b->d_len[3], b->d_work[3], b->nwork[3], 3, s);
+    const int mode = rk == 0 ? P5_RK0 : (rk < 3 ? P5_RK12 : P5_RK3);
+    launch_batch_p5<Rt>(b->lmax[4], mode, (const P5Args<Rt>*)b->d_p5[rk], b->d_len[4], b->d_work[4], b->nwork[4], s);
+  }
+  hipLaunchKernelGGL(k_batch_sum, dim3(b->nbox), dim3(256), 0, s, (const double*)b->d_part, (const int*)b->d_poff,
+                     (const int*)b->d_pcnt, (const double*)b->d_nu, b->d_diag);
+  return check_launch();
+}
+
+int batch_step_any(sdns_batch* b, hipStream_t s) {
+  return b->precision == SDNS_DOUBLE ? batch_step<double>(b, s) : batch_step<float>(b, s);
+}
+
+void batch_drop_graph(sdns_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int batch_capture(sdns_batch* b) {
+  HIP_TRY(hipStreamBeginCapture(b->own, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_step_any(b, b->own);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->own, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(SDNS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -560,7 +814,7 @@ int sdns_plan_set_stream(sdns_plan* p, void* s) {
 }
 
 size_t sdns_workspace_bytes(const sdns_plan* p) {
-  return p ? p->szA + p->szB + 2 * p->szS + p->szPart : 0;
+  return p ? p->szA + p->szB + 3 * p->szS + p->szPart : 0;
 }
 
 int sdns_set_workspace(sdns_plan* p, void* d, size_t bytes) {
@@ -648,11 +902,10 @@ int sdns_convection(sdns_plan* p, const void* u, void* rhs) {
 int sdns_rk4_steps(sdns_plan* p, void* u, void* u_lo, const double* dts, int nsteps, double nu, double* diag,
                    void* ph) {
   if (!p || !u || (!dts && nsteps > 0) || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
-  if (u_lo && p->precision == SDNS_SINGLE) return fail(SDNS_ERR_UNSUPPORTED, "compensated accumulation not built");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = p->precision == SDNS_DOUBLE ? rk4_steps<double>(p, u, dts, nsteps, nu, diag, ph)
-                                   : rk4_steps<float>(p, u, dts, nsteps, nu, diag, ph);
+  rc = p->precision == SDNS_DOUBLE ? rk4_steps<double>(p, u, nullptr, dts, nsteps, nu, diag, ph)
+                                   : rk4_steps<float>(p, u, u_lo, dts, nsteps, nu, diag, ph);
   return rc ? rc : check_launch();
 }
 
@@ -701,6 +954,124 @@ int sdns_energy(sdns_plan* p, const void* u, double nu, double* out) {
   if (rc) return rc;
   rc = p->precision == SDNS_DOUBLE ? energy<double>(p, u, nu, out) : energy<float>(p, u, nu, out);
   return rc ? rc : check_launch();
+}
+
+// ---------------------------------------------------------------- batched ensembles
+int sdns_batch_destroy(sdns_batch* b) {
+  if (!b) return 0;
+  if (b->own) (void)hipStreamSynchronize(b->own);
+  batch_drop_graph(b);
+  for (sdns_plan* p : b->box) sdns_plan_destroy(p);
+  if (b->dmem) (void)hipFree(b->dmem);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->own) (void)hipStreamDestroy(b->own);
+  delete b;
+  return 0;
+}
+
+int sdns_batch_create(sdns_batch** out, int nbox, const int32_t* N, const double* L, int precision, int dealias,
+                      int mask_nyquist, int dealias_mode, void* hip_stream) {
+  if (!out || !N || !L || nbox < 1) return fail(SDNS_ERR_INVALID, "bad argument");
+  *out = nullptr;
+  if (dealias == SDNS_DEALIAS_NONE)
+    return fail(SDNS_ERR_UNSUPPORTED, "batched ensembles run the dealiased (2/3 or 3/2 rule) step");
+  sdns_batch* b = new sdns_batch();
+  b->nbox = nbox;
+  b->precision = precision;
+  b->user = (hipStream_t)hip_stream;
+  auto bail = [&](int rc) { sdns_batch_destroy(b); return rc; };
+  if (hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(SDNS_ERR_HIP, "stream / event creation failed"));
+  for (int i = 0; i < nbox; ++i) {
+    sdns_plan* p = nullptr;
+    int rc = sdns_plan_create(&p, N + 3 * i, L + 3 * i, precision, dealias, mask_nyquist, dealias_mode, b->own);
+    if (rc) return bail(rc);
+    b->box.push_back(p);
+    if (!batch_strided_ok(p->M[0]) || !batch_strided_ok(p->M[1]) || !batch_half_ok(p->n2h))
+      return bail(fail(SDNS_ERR_UNSUPPORTED, "box %d: no batched kernel for lines %d x %d x %d (x, y in 16..128 "
+                       "with a power-of-two thread count per line, z/2 in 8..64)", i, p->M[0], p->M[1], p->M[2]));
+    if ((rc = ensure_workspace(p))) return bail(rc);
+  }
+  const int rc = precision == SDNS_DOUBLE ? batch_layout<double>(b) : batch_layout<float>(b);
+  if (rc) return bail(rc);
+  b->u.assign(nbox, nullptr);
+  b->nu.assign(nbox, 0.0);
+  b->dt.assign(nbox, 0.0);
+  *out = b;
+  return 0;
+}
+
+int sdns_batch_set_state(sdns_batch* b, void* const* u_hat) {
+  if (!b || !u_hat) return fail(SDNS_ERR_INVALID, "bad argument");
+  for (int i = 0; i < b->nbox; ++i)
+    if (!u_hat[i]) return fail(SDNS_ERR_INVALID, "u_hat[%d] is NULL", i);
+  HIP_TRY(hipStreamSynchronize(b->own));
+  for (int i = 0; i < b->nbox; ++i) b->u[i] = u_hat[i];
+  b->have_state = true;
+  b->dirty = true;
+  return 0;
+}
+
+int sdns_batch_set_params(sdns_batch* b, const double* nu, const double* dt) {
+  if (!b || !nu || !dt) return fail(SDNS_ERR_INVALID, "bad argument");
+  for (int i = 0; i < b->nbox; ++i)
+    if (!(nu[i] >= 0) || !(dt[i] > 0)) return fail(SDNS_ERR_INVALID, "box %d: nu %g, dt %g", i, nu[i], dt[i]);
+  HIP_TRY(hipStreamSynchronize(b->own));
+  for (int i = 0; i < b->nbox; ++i) { b->nu[i] = nu[i]; b->dt[i] = dt[i]; }
+  b->have_params = true;
+  b->dirty = true;
+  return 0;
+}
+
+int sdns_batch_use_graph(sdns_batch* b, int enable) {
+  if (!b) return fail(SDNS_ERR_INVALID, "null batch");
+  b->use_graph = enable != 0;
+  return 0;
+}
+
+int sdns_batch_set_stream(sdns_batch* b, void* hip_stream) {
+  if (!b) return fail(SDNS_ERR_INVALID, "null batch");
+  b->user = (hipStream_t)hip_stream;
+  return 0;
+}
+
+int sdns_batch_rk4_steps(sdns_batch* b, int nsteps, double* diag) {
+  if (!b || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (!b->have_state || !b->have_params) return fail(SDNS_ERR_INVALID, "set the state and the parameters first");
+  if (b->dirty) {
+    HIP_TRY(hipStreamSynchronize(b->own));
+    const int rc = b->precision == SDNS_DOUBLE ? batch_upload<double>(b) : batch_upload<float>(b);
+    if (rc) return rc;
+    b->dirty = false;
+  }
+  HIP_TRY(hipEventRecord(b->ev_in, b->user));
+  HIP_TRY(hipStreamWaitEvent(b->own, b->ev_in, 0));
+  if (b->use_graph && nsteps > 0 && !b->exec) {
+    const int rc = batch_capture(b);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    if (b->use_graph) {
+      HIP_TRY(hipGraphLaunch(b->exec, b->own));
+    } else {
+      const int rc = batch_step_any(b, b->own);
+      if (rc) return rc;
+    }
+  }
+  if (diag && nsteps > 0)
+    HIP_TRY(hipMemcpyAsync(diag, b->d_diag, 2 * b->nbox * sizeof(double), hipMemcpyDeviceToDevice, b->own));
+  HIP_TRY(hipEventRecord(b->ev_out, b->own));
+  HIP_TRY(hipStreamWaitEvent(b->user, b->ev_out, 0));
+  return 0;
+}
+
+int sdns_batch_box_plan(sdns_batch* b, int i, sdns_plan** plan) {
+  if (!b || !plan || i < 0 || i >= b->nbox) return fail(SDNS_ERR_INVALID, "bad argument");
+  *plan = b->box[i];
+  return 0;
 }
 
 }  // extern "C"

@@ -99,24 +99,18 @@ __device__ __forceinline__ void table_to_lds(C* dst, const C* __restrict__ src, 
 // ------------------------------------------------------------------ P1
 // blockIdx.y selects the component: y = 0 -> w (fields 2 and 3 = X(kx w)),
 // y = 1 -> v (fields 1 and 4 = X(kx v)), y = 2 -> u (field 0).
-template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(2 * E, esz_of<Rt>()))) k_p1(P1Args<Rt> a) {
+template <typename Rt, int M, int E, int T>
+__device__ __forceinline__ void p1_body(const P1Args<Rt>& a, int tile, int yb, cx<Rt>* lds, const cx<Rt>* tw) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M, E>;
-  constexpr int P = Cfg::P, T = Cfg::T;
+  constexpr int P = M / E;
   using F = LineFFT<Rt, M, E, +1>;
-  __shared__ C lds[M * T];
-  __shared__ C tw[M];
-  const int tile = xcd_tile((a.Q + T - 1) / T);
-  if (tile < 0) return;
-  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;  // compact column iyc * nkzp + kz
   const int iyc = q / a.nkzp, kzi = q - iyc * a.nkzp;
   const bool valid = q < a.Q && kzi < a.nkz;
   const int nky = a.y_neg + a.y_pos + 1;
   const int iy = iyc <= a.y_pos ? iyc : a.N1 - (nky - iyc);
-  const int comp = 2 - (int)blockIdx.y;
+  const int comp = 2 - yb;
   const long plane = (long)a.N1 * a.Nh;
   const C* __restrict__ U = (const C*)a.u + (long)comp * a.N0 * plane + (valid ? (long)iy * a.Nh + kzi : 0);
   C v[E];
@@ -149,21 +143,27 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(2 * E, esz
   store(comp, v);
 }
 
-// ------------------------------------------------------------------ P2
-// curl(u)^ = i (K x u_hat) (maths/cross.py:27-32) assembled after the
-// x-inverse: wx = i(ky W - kz V), wy = i(kz U - X(kx W)), wz = i(X(kx V) - ky U).
-// blockIdx.y = 0: V, W -> wx, V, W ; blockIdx.y = 1: U, X(kx W), X(kx V) -> wy, wz, U.
 template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p2(P2Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(2 * E, esz_of<Rt>()))) k_p1(P1Args<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M, E>;
-  constexpr int P = Cfg::P, T = Cfg::T;
-  using F = LineFFT<Rt, M, E, +1>;
+  constexpr int T = SCfg<Rt, M, E>::T;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
   if (tile < 0) return;
   table_to_lds(tw, (const C*)a.tw, M);
+  p1_body<Rt, M, E, T>(a, tile, blockIdx.y, lds, tw);
+}
+
+// ------------------------------------------------------------------ P2
+// curl(u)^ = i (K x u_hat) (maths/cross.py:27-32) assembled after the
+// x-inverse: wx = i(ky W - kz V), wy = i(kz U - X(kx W)), wz = i(X(kx V) - ky U).
+// blockIdx.y = 0: V, W -> wx, V, W ; blockIdx.y = 1: U, X(kx W), X(kx V) -> wy, wz, U.
+template <typename Rt, int M, int E, int T>
+__device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, cx<Rt>* lds, const cx<Rt>* tw) {
+  using C = cx<Rt>;
+  constexpr int P = M / E;
+  using F = LineFFT<Rt, M, E, +1>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;  // column x * nkzp + kz
   const int x = q / a.nkzp, kzi = q - x * a.nkzp;
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz
       for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * rs] = v[e];
     }
   };
-  if (blockIdx.y == 0) {
+  if (yb == 0) {
     C A[E], B[E], om[E];
     load(1, A);  // V
     load(2, B);  // W
@@ -231,6 +231,18 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz
     F::run(A, t, line, T, tw);
     store(0, A);
   }
+}
+
+template <typename Rt, int M, int E>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p2(P2Args<Rt> a) {
+  using C = cx<Rt>;
+  constexpr int T = SCfg<Rt, M, E>::T;
+  __shared__ C lds[M * T];
+  __shared__ C tw[M];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
+  p2_body<Rt, M, E, T>(a, tile, blockIdx.y, lds, tw);
 }
 
 // ------------------------------------------------------------------ z-line real transforms
@@ -377,19 +389,15 @@ template <typename Rt, int n, int E_> struct ZBCfg {
   static constexpr int THREADS = LPB * P;
 };
 
-template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3b(P3Args<Rt> a) {
+template <typename Rt, int n, int E, int LPB>
+__device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>* lds, const cx<Rt>* tw,
+                                         const cx<Rt>* rtw) {
   using C = cx<Rt>;
-  using Cfg = ZBCfg<Rt, n, E>;
-  constexpr int P = Cfg::P, LPB = Cfg::LPB, SL = n + ZPAD;
+  constexpr int P = n / E, SL = n + ZPAD;
   using Fi = LineFFT<Rt, n, E, +1>;
   using Ff = LineFFT<Rt, n, E, -1>;
-  __shared__ C lds[3 * LPB * SL];
-  __shared__ C tw[n], rtw[n];
-  table_to_lds(tw, (const C*)a.tw, n);
-  table_to_lds(rtw, (const C*)a.rtw, n);
   const int t = threadIdx.x % P, l = threadIdx.x / P;
-  const int line = blockIdx.x * LPB + l;
+  const int line = line0 + l;
   const bool valid = line < a.L;
   const long li = valid ? line : 0;
   C* s0 = lds + (3 * l) * SL;
@@ -451,6 +459,17 @@ __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + s
   }
 }
 
+template <typename Rt, int n, int E>
+__global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3b(P3Args<Rt> a) {
+  using C = cx<Rt>;
+  constexpr int LPB = ZBCfg<Rt, n, E>::LPB;
+  __shared__ C lds[3 * LPB * (n + ZPAD)];
+  __shared__ C tw[n], rtw[n];
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
+  p3b_body<Rt, n, E, LPB>(a, blockIdx.x * LPB, lds, tw, rtw);
+}
+
 // ------------------------------------------------------------------ plain z transforms
 template <typename Rt, int n, int E>
 __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_c2r(ZArgs<Rt> a) {
@@ -509,23 +528,18 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
 }
 
 // ------------------------------------------------------------------ generic strided
-template <typename Rt, int M, int E, int DIR>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(E, esz_of<Rt>()))) k_strided(StridedArgs<Rt> a) {
+template <typename Rt, int M, int E, int DIR, int T>
+__device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile, int yb, cx<Rt>* lds,
+                                             const cx<Rt>* tw) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M, E>;
-  constexpr int P = Cfg::P, T = Cfg::T;
+  constexpr int P = M / E;
   using F = LineFFT<Rt, M, E, DIR>;
-  __shared__ C lds[M * T];
-  __shared__ C tw[M];
-  const int tile = xcd_tile((a.Q + T - 1) / T);
-  if (tile < 0) return;
-  table_to_lds(tw, (const C*)a.tw, M);
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;
   const int hi = q / a.cdiv, lo = q - hi * a.cdiv;
   const bool valid = q < a.Q && lo < a.cvalid;
-  const C* __restrict__ in = (const C*)a.in + blockIdx.y * a.in_field + (valid ? hi * a.in_hi + lo : 0);
-  C* __restrict__ out = (C*)a.out + blockIdx.y * a.out_field + hi * a.out_hi + lo;
+  const C* __restrict__ in = (const C*)a.in + yb * a.in_field + (valid ? hi * a.in_hi + lo : 0);
+  C* __restrict__ out = (C*)a.out + yb * a.out_field + hi * a.out_hi + lo;
   C v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -543,19 +557,49 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(E, esz_of<
   }
 }
 
-// ------------------------------------------------------------------ P5
-template <typename Rt, int M, int E, int MODE>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p5(P5Args<Rt> a) {
+template <typename Rt, int M, int E, int DIR>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(E, esz_of<Rt>()))) k_strided(StridedArgs<Rt> a) {
   using C = cx<Rt>;
-  using Cfg = SCfg<Rt, M, E>;
-  constexpr int P = Cfg::P, T = Cfg::T, TH = Cfg::THREADS;
-  using F = LineFFT<Rt, M, E, -1>;
+  constexpr int T = SCfg<Rt, M, E>::T;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
-  const int ntiles = (a.Q + T - 1) / T;
-  const int tile = xcd_tile(ntiles);
+  const int tile = xcd_tile((a.Q + T - 1) / T);
   if (tile < 0) return;
   table_to_lds(tw, (const C*)a.tw, M);
+  strided_body<Rt, M, E, DIR, T>(a, tile, blockIdx.y, lds, tw);
+}
+
+// ------------------------------------------------------------------ P5
+// Knuth's TwoSum: returns s = fl(a + b) and sets err so that s + err = a + b
+// exactly (no contraction inside).
+template <typename Rt> __device__ __forceinline__ Rt two_sum(Rt a, Rt b, Rt& err) {
+#pragma clang fp contract(off)
+  const Rt s = a + b;
+  const Rt bp = s - a;
+  err = (a - (s - bp)) + (b - bp);
+  return s;
+}
+
+// w |u|^2 and w |K x u|^2 of one mode (components x0, y0, x1, y1, x2, y2) in FP64.
+template <typename Rt>
+__device__ __forceinline__ void energy_terms(const double (&v)[6], Rt K0, Rt K1, Rt K2, double w, double& esum,
+                                             double& osum) {
+  esum += w * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+  const double k0 = K0, k1 = K1, k2 = K2;
+  const double ax = k1 * v[4] - k2 * v[2], ay = k1 * v[5] - k2 * v[3];
+  const double bx = k2 * v[0] - k0 * v[4], by = k2 * v[1] - k0 * v[5];
+  const double cx = k0 * v[2] - k1 * v[0], cy = k0 * v[3] - k1 * v[1];
+  osum += w * (ax * ax + ay * ay + bx * bx + by * by + cx * cx + cy * cy);
+}
+
+// COMP: compensated accumulation of the RK4 sum u2 = u1 + sum a dt rhs in a
+// (hi, lo) pair of single-precision fields; the state is u = hi + lo with hi
+// the caller's u_hat and lo the caller's u_lo.  Stage values stay plain.
+template <typename Rt, int M, int E, int MODE, bool COMP, int T>
+__device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* lds, const cx<Rt>* tw) {
+  using C = cx<Rt>;
+  constexpr int P = M / E, TH = T * P;
+  using F = LineFFT<Rt, M, E, -1>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
   const int q = tile * T + c;
   const bool valid = q < a.Q;
@@ -646,29 +690,75 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz
       C* Cc = (C*)a.Cacc;
       const Rt bdt = a.bdt, adt = a.adt;
       auto upd = [](C base, C r, Rt s) { return mk<C>(base.x + s * r.x, base.y + s * r.y); };
-      if constexpr (MODE == P5_RK0) {
-        // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
-        B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
-        Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
-      } else if constexpr (MODE == P5_RK12) {
-        const C b0 = A[off], b1 = A[fs + off], b2 = A[2 * fs + off];
-        B[off] = upd(b0, r0, bdt); B[fs + off] = upd(b1, r1, bdt); B[2 * fs + off] = upd(b2, r2, bdt);
-        const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
-        Cc[off] = upd(c0, r0, adt); Cc[fs + off] = upd(c1, r1, adt); Cc[2 * fs + off] = upd(c2, r2, adt);
+      if constexpr (!COMP) {
+        if constexpr (MODE == P5_RK0) {
+          // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
+          B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
+          Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
+        } else if constexpr (MODE == P5_RK12) {
+          const C b0 = A[off], b1 = A[fs + off], b2 = A[2 * fs + off];
+          B[off] = upd(b0, r0, bdt); B[fs + off] = upd(b1, r1, bdt); B[2 * fs + off] = upd(b2, r2, bdt);
+          const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
+          Cc[off] = upd(c0, r0, adt); Cc[fs + off] = upd(c1, r1, adt); Cc[2 * fs + off] = upd(c2, r2, adt);
+        } else {
+          const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
+          const C n0 = upd(c0, r0, adt), n1 = upd(c1, r1, adt), n2 = upd(c2, r2, adt);
+          A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
+          if (a.phat) ((C*)a.phat)[off] = ph;
+          if (a.partials) {
+            const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+            const double v[6] = {n0.x, n0.y, n1.x, n1.y, n2.x, n2.y};
+            energy_terms(v, K0, K1, K2c, w, esum, osum);
+          }
+        }
       } else {
-        const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
-        const C n0 = upd(c0, r0, adt), n1 = upd(c1, r1, adt), n2 = upd(c2, r2, adt);
-        A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
-        if (a.phat) ((C*)a.phat)[off] = ph;
-        if (a.partials) {
-          const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
-          const double x0 = n0.x, y0 = n0.y, x1 = n1.x, y1 = n1.y, x2 = n2.x, y2 = n2.y;
-          esum += w * (x0 * x0 + y0 * y0 + x1 * x1 + y1 * y1 + x2 * x2 + y2 * y2);
-          const double k0 = K0, k1 = K1, kk2 = K2c;
-          const double ax = k1 * x2 - kk2 * x1, ay = k1 * y2 - kk2 * y1;
-          const double bx = kk2 * x0 - k0 * x2, by = kk2 * y0 - k0 * y2;
-          const double cxv = k0 * x1 - k1 * x0, cyv = k0 * y1 - k1 * y0;
-          osum += w * (ax * ax + ay * ay + bx * bx + by * by + cxv * cxv + cyv * cyv);
+        C* Alo = (C*)a.Alo;
+        C* Clo = (C*)a.Clo;
+        const C rr[3] = {r0, r1, r2};
+        if constexpr (MODE == P5_RK0) {
+          const C ss[3] = {s0, s1, s2};
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const long o = i * fs + off;
+            B[o] = upd(ss[i], rr[i], bdt);
+            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+            Rt ex, ey;
+            const Rt hx = two_sum(ss[i].x, yx, ex), hy = two_sum(ss[i].y, yy, ey);
+            const C l = Alo[o];
+            Cc[o] = mk<C>(hx, hy);
+            Clo[o] = mk<C>(l.x + ex, l.y + ey);
+          }
+        } else if constexpr (MODE == P5_RK12) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const long o = i * fs + off;
+            B[o] = upd(A[o], rr[i], bdt);
+            const C c = Cc[o], l = Clo[o];
+            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+            Rt ex, ey;
+            const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
+            Cc[o] = mk<C>(hx, hy);
+            Clo[o] = mk<C>(l.x + ex, l.y + ey);
+          }
+        } else {
+          double v[6];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const long o = i * fs + off;
+            const C c = Cc[o], l = Clo[o];
+            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+            Rt ex, ey;
+            const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
+            const Rt lx = l.x + ex, ly = l.y + ey;
+            Rt rx, ry;  // renormalise: hi = fl(h + l), lo = exact remainder
+            const Rt nx = two_sum(hx, lx, rx), ny = two_sum(hy, ly, ry);
+            A[o] = mk<C>(nx, ny);
+            Alo[o] = mk<C>(rx, ry);
+            v[2 * i] = (double)nx + (double)rx;
+            v[2 * i + 1] = (double)ny + (double)ry;
+          }
+          if (a.phat) ((C*)a.phat)[off] = ph;
+          if (a.partials) energy_terms(v, K0, K1, K2c, (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0, esum, osum);
         }
       }
     }
@@ -692,6 +782,18 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz
       }
     }
   }
+}
+
+template <typename Rt, int M, int E, int MODE, bool COMP = false>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p5(P5Args<Rt> a) {
+  using C = cx<Rt>;
+  constexpr int T = SCfg<Rt, M, E>::T;
+  __shared__ C lds[M * T];
+  __shared__ C tw[M];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
+  p5_body<Rt, M, E, MODE, COMP, T>(a, tile, lds, tw);
 }
 
 }  // namespace sdns

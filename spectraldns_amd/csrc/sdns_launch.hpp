@@ -58,4 +58,67 @@ template <typename Rt> int launch_force_scale(const EwArgs<Rt>& a, Rt kf2, const
 template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* partials, int nblk, hipStream_t s);
 int launch_sum_rows(const double* partials, int nblk, int width, double* out, hipStream_t s);
 
+// ------------------------------------------------------------------ batched ensembles
+// Line lengths of the batched pass kernels (strided x/y lines; half-length z
+// lines).  Every block of a batched launch has the same size whatever the
+// box, so the tile per length is fixed: T = threads * E / M columns for the
+// strided passes, LPB = zthreads * E / n lines for the z pass.
+#define SDNS_BATCH_STRIDED(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128)
+#define SDNS_BATCH_HALF(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+
+constexpr int batch_E(int M) {
+  switch (M) {
+    case 16: return 4;  case 24: return 6;  case 32: return 8;  case 48: return 12;
+    case 64: return 8;  case 96: return 12; case 128: return 16;
+    default: return 0;
+  }
+}
+constexpr int batch_zE(int n) {
+  switch (n) {
+    case 8: return 8; case 12: return 6; case 16: return 8; case 24: return 6;
+    case 32: return 8; case 48: return 6; case 64: return 8;
+    default: return 0;
+  }
+}
+template <typename Rt> constexpr int batch_threads() { return sizeof(Rt) == 4 ? 256 : 128; }
+template <typename Rt> constexpr int batch_zthreads() { return sizeof(Rt) == 4 ? 128 : 64; }
+template <typename Rt> constexpr int batch_T(int M) {
+  return batch_E(M) ? batch_threads<Rt>() * batch_E(M) / M : 0;
+}
+template <typename Rt> constexpr int batch_LPB(int n) {
+  return batch_zE(n) ? batch_zthreads<Rt>() * batch_zE(n) / n : 0;
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// LDS (complex elements) of a strided batch kernel compiled for lengths up
+// to lmax: tile + twiddle table
+template <typename Rt> constexpr int batch_strided_lds(int lmax) {
+  int m = 0;
+#define SDNS_X(M) if (M <= lmax) m = cmax(m, M * batch_T<Rt>(M) + M);
+  SDNS_BATCH_STRIDED(SDNS_X)
+#undef SDNS_X
+  return m;
+}
+// LDS of the z batch kernel: three padded line slots per line + two tables
+template <typename Rt> constexpr int batch_z_lds(int lmax, int zpad) {
+  int m = 0;
+#define SDNS_X(n) if (n <= lmax) m = cmax(m, 3 * batch_LPB<Rt>(n) * (n + zpad) + 2 * n);
+  SDNS_BATCH_HALF(SDNS_X)
+#undef SDNS_X
+  return m;
+}
+
+// args / len / work are device arrays: one argument block and one line length
+// per box, one work item per block (nwork blocks, a multiple of 8); lmax is
+// the largest line length of the ensemble for the pass.
+template <typename Rt> int launch_batch_p1(int lmax, const P1Args<Rt>* args, const int* len, const BatchWork* work,
+                                           int nwork, hipStream_t s);
+template <typename Rt> int launch_batch_p2(int lmax, const P2Args<Rt>* args, const int* len, const BatchWork* work,
+                                           int nwork, hipStream_t s);
+template <typename Rt> int launch_batch_p3(int lmax, const P3Args<Rt>* args, const int* len, const BatchWork* work,
+                                           int nwork, hipStream_t s);
+template <typename Rt> int launch_batch_fwd(int lmax, const StridedArgs<Rt>* args, const int* len,
+                                            const BatchWork* work, int nwork, int nfields, hipStream_t s);
+template <typename Rt> int launch_batch_p5(int lmax, int mode, const P5Args<Rt>* args, const int* len,
+                                           const BatchWork* work, int nwork, hipStream_t s);
+
 }  // namespace sdns

@@ -30,6 +30,16 @@ template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, h
   const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
   if (nblk) *nblk = nt;
   const dim3 g(xcd_grid(nt)), b(Cfg::THREADS);
+  if constexpr (sizeof(Rt) == 4) {
+    if (a.Alo && mode != P5_RHS) {
+      switch (mode) {
+        case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK0, true>), g, b, 0, s, a); break;
+        case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK12, true>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK3, true>), g, b, 0, s, a); break;
+      }
+      return;
+    }
+  }
   switch (mode) {
     case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RHS>), g, b, 0, s, a); break;
     case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK0>), g, b, 0, s, a); break;

@@ -94,6 +94,8 @@ template <typename Rt> struct P5Args {  // x-forward + mask + pressure + diffusi
   void* A;            // step base u1 (and final result)
   void* B;            // stage value u0
   void* Cacc;         // accumulator u2
+  void* Alo;          // compensated mode: low part of u1 / of the result (caller's u_lo)
+  void* Clo;          // compensated mode: low part of u2
   void* rhs;          // P5_RHS: rhs output
   void* phat;         // P5_RHS: pressure output (may be null)
   const void* source; // P5_RHS: added source (may be null)
@@ -108,6 +110,12 @@ template <typename Rt> struct P5Args {  // x-forward + mask + pressure + diffusi
   int conv_only;      // P5_RHS: write the scaled convection term only
   Rt nu, scale, adt, bdt;
   int Q;              // N1 * Nh state columns
+};
+
+// One block of a batched pass: box index (-1 for padding blocks) and the
+// block's tile within that box.
+struct BatchWork {
+  int box, tile;
 };
 
 }  // namespace sdns

@@ -15,9 +15,13 @@ __all__ = ["getintegrator", "RK4"]
 
 
 def RK4(u0, u1, u2, rhs, a, b, dt, solver, context):
-    """One classical RK4 step on u0 in place; returns (u0, dt, dt)."""
+    """One classical RK4 step on u0 in place; returns (u0, dt, dt).
+
+    With params.compensated in single precision the state is carried as
+    u0 + context.U_hat_lo (a compensated pair, see sdns_rk4_steps)."""
     plan = context.VT.plan
-    plan.rk4_steps(u0, [dt], solver.params.nu, p_hat=context.P_hat)
+    lo = context.get("U_hat_lo") if solver.params.get("compensated", False) else None
+    plan.rk4_steps(u0, [dt], solver.params.nu, p_hat=context.P_hat, u_lo=lo)
     return u0, dt, dt
 
 
@@ -31,6 +35,9 @@ def getintegrator(rhs, u0, solver, context):
         raise NotImplementedError("the fused RK4 step serves the Vortex convection of NS")
     if solver.ComputeRHS is not getattr(solver, "_ComputeRHS_fused", None):
         raise NotImplementedError("a replaced ComputeRHS cannot run inside the fused RK4 step")
+    if params.get("compensated", False) and params.precision == "single" and "U_hat_lo" not in context:
+        import torch
+        context["U_hat_lo"] = torch.zeros_like(u0)
     a = np.array([1. / 6., 1. / 3., 1. / 3., 1. / 6.], dtype=context.float)
     b = np.array([0.5, 0.5, 1.], dtype=context.float)
 

@@ -172,6 +172,50 @@ int sdns_batch_rk4_steps(sdns_batch* batch, int nsteps, double* diag);
  * sdns_energy or sdns_spectrum on that box). */
 int sdns_batch_box_plan(sdns_batch* batch, int i, sdns_plan** plan);
 
+/* ------------------------------------------------------------------------
+ * Learned subgrid closure.  No counterpart in the reference (its solvers run
+ * resolved DNS only); the coarse-grid step it serves is the reference's RK4
+ * (integrators.py:149-159) with a Source term (NS.py:256-258) recomputed from
+ * every stage: Source = P(-i k_j tau_ij), tau predicted from the filtered
+ * velocity by a network of three periodic 3x3x3 conv layers 3 -> 32 -> 32 -> 6
+ * (bias, SiLU after layers 1 and 2), bf16 weights and activations, FP32
+ * accumulation on the matrix cores.
+ *
+ * Parameters are one FP32 device vector of sdns_closure_nparams() entries:
+ * W1[27][32][3] b1[32] W2[27][32][32] b2[32] W3[27][6][32] b3[6], W[tap][cout][cin]
+ * with tap = 9 (dz+1) + 3 (dy+1) + (dx+1) applied to the input at the point
+ * (i0+dz, i1+dy, i2+dx) (periodic).  tau components: xx, yy, zz, xy, xz, yz.
+ * Physical fields are planar FP32 device arrays [c][N0][N1][N2].
+ * Grid: N2 % 32 == 0 and N0*N1*N2 % 4096 == 0. */
+typedef struct sdns_closure sdns_closure;
+size_t sdns_closure_nparams(void);
+int sdns_closure_create(sdns_closure** closure, const int32_t N[3], void* hip_stream);
+int sdns_closure_destroy(sdns_closure* closure);
+int sdns_closure_set_stream(sdns_closure* closure, void* hip_stream);
+/* Convert the FP32 parameter vector to the packed bf16 kernel layouts; call
+ * again after every parameter update. */
+int sdns_closure_set_params(sdns_closure* closure, const float* params);
+/* tau (6 fields) = out_scale * net(in_scale * u) for u (3 fields). */
+int sdns_closure_forward(sdns_closure* closure, const float* u_phys, float in_scale, float out_scale,
+                         float* tau_phys);
+/* One training evaluation: loss = mean over points and the 6 components of
+ * (net(in_scale u) - tau / out_scale)^2 into *loss (device double, may be
+ * NULL) and its gradient w.r.t. every parameter into grad (FP32, nparams). */
+int sdns_closure_train_step(sdns_closure* closure, const float* u_phys, const float* tau_phys, float in_scale,
+                            float out_scale, float* grad, double* loss);
+/* Adam update (Kingma & Ba) of n FP32 parameters at step t >= 1 (device arrays). */
+int sdns_adam_step(float* params, const float* grad, float* m, float* v, size_t n, float lr, float beta1,
+                   float beta2, float eps, int t, void* hip_stream);
+/* Box filter by r of the physical velocity u (3 fields, plan's grid and
+ * precision): ubar (3 fields) = means over non-overlapping r^3 blocks, tau (6
+ * fields) = bar(u_i u_j) - ubar_i ubar_j with FP64 sums. */
+int sdns_box_filter(sdns_plan* plan, const void* u_phys, int r, void* ubar, void* tau);
+/* sdns_rk4_steps on a single-precision plan of the closure's grid with the
+ * closure's Source added at every stage (the stage state's velocity is
+ * transformed back, evaluated by the network, tau transformed forward). */
+int sdns_closure_rk4_steps(sdns_closure* closure, sdns_plan* plan, void* u_hat, const double* dt_seq, int nsteps,
+                           double nu, float in_scale, float out_scale, double* diag);
+
 /* Human-readable message of the last failing call on this thread. */
 const char* sdns_last_error(void);
 

@@ -52,6 +52,18 @@ _SIGS = {
     "sdns_batch_set_stream": (ctypes.c_int, [_P, _P]),
     "sdns_batch_rk4_steps": (ctypes.c_int, [_P, ctypes.c_int, _P]),
     "sdns_batch_box_plan": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P)]),
+    "sdns_closure_nparams": (ctypes.c_size_t, []),
+    "sdns_closure_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32), _P]),
+    "sdns_closure_destroy": (ctypes.c_int, [_P]),
+    "sdns_closure_set_stream": (ctypes.c_int, [_P, _P]),
+    "sdns_closure_set_params": (ctypes.c_int, [_P, _P]),
+    "sdns_closure_forward": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, _P]),
+    "sdns_closure_train_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, ctypes.c_float, _P, _P]),
+    "sdns_adam_step": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_size_t, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_int, _P]),
+    "sdns_box_filter": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, _P]),
+    "sdns_closure_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_float, ctypes.c_float, _P]),
     "sdns_last_error": (ctypes.c_char_p, []),
 }
 
@@ -287,6 +299,19 @@ class Plan:
         self._sync_stream()
         _check(self._lib.sdns_spectrum(self._h, _ptr(u_hat), int(nbins), _ptr(out)))
         return out[:nbins], out[nbins:]
+
+    def box_filter(self, u, r):
+        """(ubar (3, N/r...), tau (6, N/r...)) of the physical velocity u (block means, FP64 sums)."""
+        import torch
+        self._phys(u, "u")
+        if u.shape[0] != 3 or any(n % r for n in self.N):
+            raise SdnsError("box_filter: u must have 3 components and r must divide N")
+        shp = tuple(n // r for n in self.N)
+        ub = torch.empty((3,) + shp, dtype=self.real_dtype, device=self.device)
+        tau = torch.empty((6,) + shp, dtype=self.real_dtype, device=self.device)
+        self._sync_stream()
+        _check(self._lib.sdns_box_filter(self._h, _ptr(u), int(r), _ptr(ub), _ptr(tau)))
+        return ub, tau
 
     def set_variant(self, pass_id, variant):
         """Schedule variant (0 default, 1 alternative) of pass 0..4, 5 = plain transforms."""

@@ -12,6 +12,7 @@
 
 #include "../../include/sdns.h"
 #include "sdns_launch.hpp"
+#include "sdns_closure.hpp"
 
 using namespace sdns;
 
@@ -106,6 +107,22 @@ struct sdns_plan {
   std::vector<hipEvent_t> pool;
   int cur_id = 0;
   hipEvent_t cur_a = nullptr;
+};
+
+// Learned-closure workspace of one coarse grid (kernels in sdns_closure.hip).
+struct sdns_closure {
+  int n[3] = {0, 0, 0};
+  long P = 0;
+  hipStream_t stream = nullptr;
+  char* mem = nullptr;
+  __bf16 *x0 = nullptr, *z1 = nullptr, *a1 = nullptr, *z2 = nullptr, *a2 = nullptr;
+  __bf16 *gyb = nullptr, *gz2 = nullptr, *gz1 = nullptr;
+  float *y = nullptr, *gy = nullptr, *t = nullptr, *wpart = nullptr, *bpart = nullptr;
+  float *phys = nullptr, *tauphys = nullptr;
+  char *tauhat = nullptr, *src = nullptr;  // 6 and 3 complex64 spectral fields
+  double *lpart = nullptr, *loss = nullptr;
+  sdns::ClosureWeights w{};
+  bool have_params = false;
 };
 
 namespace {
@@ -383,8 +400,41 @@ template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs) {
   return 0;
 }
 
+template <typename Rt> EwArgs<Rt> ew_args(sdns_plan* p);
+
+// Network forward on the closure's packed input x0 -> y (FP32, 6 of 8 channels).
+int closure_net_forward(sdns_closure* c, hipStream_t st) {
+  ConvArgs a{};
+  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];
+  a.x = c->x0; a.w = c->w.w1f; a.bias = c->w.b1; a.z = c->z1; a.a = c->a1;
+  if (launch_conv(8, 32, CONV_EPI_SILU, a, st)) return fail(SDNS_ERR_UNSUPPORTED, "closure grid");
+  a.x = c->a1; a.w = c->w.w2f; a.bias = c->w.b2; a.z = c->z2; a.a = c->a2;
+  launch_conv(32, 32, CONV_EPI_SILU, a, st);
+  a.x = c->a2; a.w = c->w.w3f; a.bias = c->w.b3; a.z = nullptr; a.a = nullptr; a.y = c->y; a.ystride = 8; a.ncout = 6;
+  launch_conv(32, 16, CONV_EPI_LINEAR, a, st);
+  return 0;
+}
+
+// Projected, dealiased SGS force -i k_j tau_ij of the spectral state u (single
+// precision plan on the closure's grid) into c->src.
+int closure_source(sdns_closure* c, sdns_plan* p, const void* u, float in_scale, float out_scale) {
+  int rc;
+  if ((rc = plain_backward<float>(p, u, c->phys, 3))) return rc;
+  launch_pack_input(c->phys, in_scale, c->x0, c->P, p->stream);
+  if ((rc = closure_net_forward(c, p->stream))) return rc;
+  launch_unpack_output(c->y, out_scale, c->tauphys, c->P, p->stream);
+  const size_t fbytes = (size_t)p->N[0] * p->N[1] * p->Nh * p->esz;
+  if ((rc = plain_forward<float>(p, c->tauphys, c->tauhat, 3))) return rc;
+  if ((rc = plain_forward<float>(p, c->tauphys + 3 * c->P, c->tauhat + 3 * fbytes, 3))) return rc;
+  EwArgs<float> e = ew_args<float>(p);
+  e.a = c->tauhat; e.out = c->src;
+  if (launch_elementwise<float>(EW_SGS_FORCE, e, p->stream)) return fail(SDNS_ERR_HIP, "sgs force");
+  return 0;
+}
+
 template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const double* dts, int nsteps, double nu,
-                                      double* diag, void* ph) {
+                                      double* diag, void* ph, sdns_closure* cl = nullptr, float in_scale = 1.f,
+                                      float out_scale = 1.f) {
   // a, b and dt in the working precision (integrators.py:185-186, config.py:123-127)
   const Rt a[4] = {Rt(1. / 6.), Rt(1. / 3.), Rt(1. / 3.), Rt(1. / 6.)};
   const Rt b[3] = {Rt(0.5), Rt(0.5), Rt(1.)};
@@ -393,10 +443,13 @@ template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const dou
     const Rt dt = (Rt)dts[s];
     for (int rk = 0; rk < 4; ++rk) {
       int rc;
-      if ((rc = conv_passes<Rt>(p, rk == 0 ? u : (const void*)p->stB))) return rc;
+      const void* stage_in = rk == 0 ? u : (const void*)p->stB;
+      if (cl && (rc = closure_source(cl, p, stage_in, in_scale, out_scale))) return rc;
+      if ((rc = conv_passes<Rt>(p, stage_in))) return rc;
       P5Args<Rt> pa = p5_args<Rt>(p, nu);
       pa.A = u; pa.B = p->stB; pa.Cacc = p->stC;
       if (ulo) { pa.Alo = ulo; pa.Clo = p->stClo; }
+      if (cl) pa.source = cl->src;
       pa.adt = a[rk] * dt;
       pa.bdt = rk < 3 ? b[rk] * dt : Rt(0);
       const int mode = rk == 0 ? P5_RK0 : (rk < 3 ? P5_RK12 : P5_RK3);
@@ -423,6 +476,7 @@ template <typename Rt> EwArgs<Rt> ew_args(sdns_plan* p) {
   a.N0 = p->N[0]; a.N1 = p->N[1]; a.Nh = p->Nh; a.N2 = p->N[2];
   a.n = (long)p->N[0] * p->N[1] * p->Nh;
   a.mask_nyquist = p->mask_nyquist;
+  for (int i = 0; i < 3; ++i) a.keep[i] = std::max(p->in[i][0], p->in[i][1]);
   return a;
 }
 
@@ -1072,6 +1126,147 @@ int sdns_batch_box_plan(sdns_batch* b, int i, sdns_plan** plan) {
   if (!b || !plan || i < 0 || i >= b->nbox) return fail(SDNS_ERR_INVALID, "bad argument");
   *plan = b->box[i];
   return 0;
+}
+
+// ---------------------------------------------------------------- learned closure
+size_t sdns_closure_nparams(void) { return (size_t)CL_NPARAM; }
+
+int sdns_closure_destroy(sdns_closure* c) {
+  if (!c) return 0;
+  if (c->mem) (void)hipFree(c->mem);
+  delete c;
+  return 0;
+}
+
+int sdns_closure_create(sdns_closure** out, const int32_t N[3], void* hip_stream) {
+  if (!out || !N) return fail(SDNS_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const long P = (long)N[0] * N[1] * N[2];
+  if (N[0] < 4 || N[1] < 4 || N[2] % 32 || P % 4096)
+    return fail(SDNS_ERR_UNSUPPORTED, "closure grid %dx%dx%d: N2 %% 32 == 0 and N0 N1 N2 %% 4096 == 0 required",
+                N[0], N[1], N[2]);
+  sdns_closure* c = new sdns_closure();
+  for (int i = 0; i < 3; ++i) c->n[i] = N[i];
+  c->P = P;
+  c->stream = (hipStream_t)hip_stream;
+  const long nchunk = P / 4096;
+  const size_t fbytes = (size_t)N[0] * N[1] * (N[2] / 2 + 1) * 8;  // complex64 spectral field
+  size_t off = 0;
+  auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return o; };
+  const size_t o_x0 = take(P * 8 * 2), o_z1 = take(P * 64), o_a1 = take(P * 64), o_z2 = take(P * 64),
+               o_a2 = take(P * 64), o_gyb = take(P * 16), o_gz2 = take(P * 64), o_gz1 = take(P * 64),
+               o_y = take(P * 32), o_gy = take(P * 32), o_t = take(P * 32),
+               o_wp = take(nchunk * 27 * 32 * 32 * 4), o_bp = take(256 * 32 * 4), o_ph = take(3 * P * 4),
+               o_tp = take(6 * P * 4), o_th = take(6 * fbytes), o_src = take(3 * fbytes), o_lp = take(256 * 8),
+               o_l = take(8), o_w1 = take(27 * 32 * 8 * 2), o_w2 = take(27 * 32 * 32 * 2),
+               o_w3 = take(27 * 16 * 32 * 2), o_w3b = take(27 * 32 * 8 * 2), o_w2b = take(27 * 32 * 32 * 2),
+               o_b = take(80 * 4);
+  if (hipMalloc((void**)&c->mem, off) != hipSuccess) {
+    delete c;
+    return fail(SDNS_ERR_NOMEM, "hipMalloc of %zu bytes failed", off);
+  }
+  char* m = c->mem;
+  c->x0 = (__bf16*)(m + o_x0); c->z1 = (__bf16*)(m + o_z1); c->a1 = (__bf16*)(m + o_a1);
+  c->z2 = (__bf16*)(m + o_z2); c->a2 = (__bf16*)(m + o_a2); c->gyb = (__bf16*)(m + o_gyb);
+  c->gz2 = (__bf16*)(m + o_gz2); c->gz1 = (__bf16*)(m + o_gz1);
+  c->y = (float*)(m + o_y); c->gy = (float*)(m + o_gy); c->t = (float*)(m + o_t);
+  c->wpart = (float*)(m + o_wp); c->bpart = (float*)(m + o_bp); c->phys = (float*)(m + o_ph);
+  c->tauphys = (float*)(m + o_tp); c->tauhat = m + o_th; c->src = m + o_src;
+  c->lpart = (double*)(m + o_lp); c->loss = (double*)(m + o_l);
+  c->w.w1f = (__bf16*)(m + o_w1); c->w.w2f = (__bf16*)(m + o_w2); c->w.w3f = (__bf16*)(m + o_w3);
+  c->w.w3b = (__bf16*)(m + o_w3b); c->w.w2b = (__bf16*)(m + o_w2b);
+  c->w.b1 = (float*)(m + o_b); c->w.b2 = c->w.b1 + 32; c->w.b3 = c->w.b2 + 32;
+  *out = c;
+  return 0;
+}
+
+int sdns_closure_set_stream(sdns_closure* c, void* hip_stream) {
+  if (!c) return fail(SDNS_ERR_INVALID, "null closure");
+  c->stream = (hipStream_t)hip_stream;
+  return 0;
+}
+
+int sdns_closure_set_params(sdns_closure* c, const float* params) {
+  if (!c || !params) return fail(SDNS_ERR_INVALID, "bad argument");
+  launch_pack_weights(params, c->w, c->stream);
+  c->have_params = true;
+  return check_launch();
+}
+
+int sdns_closure_forward(sdns_closure* c, const float* u_phys, float in_scale, float out_scale, float* tau_phys) {
+  if (!c || !u_phys || !tau_phys) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (!c->have_params) return fail(SDNS_ERR_INVALID, "closure parameters not set");
+  launch_pack_input(u_phys, in_scale, c->x0, c->P, c->stream);
+  int rc = closure_net_forward(c, c->stream);
+  if (rc) return rc;
+  launch_unpack_output(c->y, out_scale, tau_phys, c->P, c->stream);
+  return check_launch();
+}
+
+int sdns_closure_train_step(sdns_closure* c, const float* u_phys, const float* tau_phys, float in_scale,
+                            float out_scale, float* grad, double* loss) {
+  if (!c || !u_phys || !tau_phys || !grad) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (!c->have_params) return fail(SDNS_ERR_INVALID, "closure parameters not set");
+  hipStream_t st = c->stream;
+  const long P = c->P;
+  const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+  launch_pack_input(u_phys, in_scale, c->x0, P, st);
+  int rc = closure_net_forward(c, st);
+  if (rc) return rc;
+  launch_pack_target(tau_phys, 1.0f / out_scale, c->t, P, st);
+  launch_loss(c->y, c->t, c->gy, c->gyb, P, c->lpart, c->loss, st);
+  // layer 3
+  int nch = launch_wgrad(8, 32, c->gyb, c->a2, c->wpart, n0, n1, n2, st);
+  if (nch < 0) return fail(SDNS_ERR_UNSUPPORTED, "closure grid");
+  launch_wgrad_reduce(c->wpart, nch, 8, 32, 6, 32, grad + CL_OW3, st);
+  launch_bgrad_f32(c->gy, 8, 6, P, c->bpart, grad + CL_OB3, st);
+  ConvArgs a{};
+  a.n0 = n0; a.n1 = n1; a.n2 = n2;
+  a.x = c->gyb; a.w = c->w.w3b; a.zprev = c->z2; a.a = c->gz2;
+  launch_conv(8, 32, CONV_EPI_DSILU, a, st);
+  // layer 2
+  nch = launch_wgrad(32, 32, c->gz2, c->a1, c->wpart, n0, n1, n2, st);
+  launch_wgrad_reduce(c->wpart, nch, 32, 32, 32, 32, grad + CL_OW2, st);
+  launch_bgrad_bf16(c->gz2, 32, 32, P, c->bpart, grad + CL_OB2, st);
+  a.x = c->gz2; a.w = c->w.w2b; a.zprev = c->z1; a.a = c->gz1;
+  launch_conv(32, 32, CONV_EPI_DSILU, a, st);
+  // layer 1
+  nch = launch_wgrad(32, 8, c->gz1, c->x0, c->wpart, n0, n1, n2, st);
+  launch_wgrad_reduce(c->wpart, nch, 32, 8, 32, 3, grad + CL_OW1, st);
+  launch_bgrad_bf16(c->gz1, 32, 32, P, c->bpart, grad + CL_OB1, st);
+  if (loss) HIP_TRY(hipMemcpyAsync(loss, c->loss, sizeof(double), hipMemcpyDeviceToDevice, st));
+  return check_launch();
+}
+
+int sdns_adam_step(float* p, const float* g, float* m, float* v, size_t n, float lr, float beta1, float beta2,
+                   float eps, int t, void* hip_stream) {
+  if (!p || !g || !m || !v || t < 1) return fail(SDNS_ERR_INVALID, "bad argument");
+  launch_adam(p, g, m, v, (long)n, lr, beta1, beta2, eps, t, (hipStream_t)hip_stream);
+  return check_launch();
+}
+
+int sdns_box_filter(sdns_plan* p, const void* u_phys, int r, void* ubar, void* tau) {
+  if (!p || !u_phys || !ubar || !tau) return fail(SDNS_ERR_INVALID, "bad argument");
+  const int rc = p->precision == SDNS_DOUBLE
+                     ? launch_box_filter_f64((const double*)u_phys, p->N[0], p->N[1], p->N[2], r, (double*)ubar,
+                                             (double*)tau, p->stream)
+                     : launch_box_filter((const float*)u_phys, p->N[0], p->N[1], p->N[2], r, (float*)ubar,
+                                         (float*)tau, p->stream);
+  if (rc) return fail(SDNS_ERR_INVALID, "filter width %d does not divide the grid", r);
+  return check_launch();
+}
+
+int sdns_closure_rk4_steps(sdns_closure* c, sdns_plan* p, void* u, const double* dts, int nsteps, double nu,
+                           float in_scale, float out_scale, double* diag) {
+  if (!c || !p || !u || (!dts && nsteps > 0) || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (!c->have_params) return fail(SDNS_ERR_INVALID, "closure parameters not set");
+  if (p->precision != SDNS_SINGLE) return fail(SDNS_ERR_UNSUPPORTED, "the closure runs in single precision");
+  for (int i = 0; i < 3; ++i)
+    if (p->N[i] != c->n[i]) return fail(SDNS_ERR_INVALID, "plan and closure grids differ");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = rk4_steps<float>(p, u, nullptr, dts, nsteps, nu, diag, nullptr, c, in_scale, out_scale);
+  return rc ? rc : check_launch();
 }
 
 }  // extern "C"

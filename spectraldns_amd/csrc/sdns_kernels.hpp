@@ -676,12 +676,12 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
     r0 = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
     r1 = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
     r2 = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
+    if (a.source) {  // rhs += Source (NS.py:256-258), every mode
+      const C* sr = (const C*)a.source;
+      r0 = cadd(r0, sr[off]); r1 = cadd(r1, sr[fs + off]); r2 = cadd(r2, sr[2 * fs + off]);
+    }
     if constexpr (MODE == P5_RHS) {
       C* rhs = (C*)a.rhs;
-      if (a.source) {
-        const C* sr = (const C*)a.source;
-        r0 = cadd(r0, sr[off]); r1 = cadd(r1, sr[fs + off]); r2 = cadd(r2, sr[2 * fs + off]);
-      }
       rhs[off] = r0; rhs[fs + off] = r1; rhs[2 * fs + off] = r2;
       if (a.phat) ((C*)a.phat)[off] = ph;
     } else {

@@ -211,6 +211,27 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
       O[n + i] = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
       O[2 * n + i] = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
       if (a.aux) ((C*)a.aux)[i] = ph;
+    } else if constexpr (OP == EW_SGS_FORCE) {
+      const int kx = ix < a.N0 / 2 ? ix : ix - a.N0, ky = iy < a.N1 / 2 ? iy : iy - a.N1;
+      const bool keep = (kx < 0 ? -kx : kx) <= a.keep[0] && (ky < 0 ? -ky : ky) <= a.keep[1] && kz <= a.keep[2] &&
+                        !(a.mask_nyquist && (ix == a.N0 / 2 || iy == a.N1 / 2 || kz == a.N2 / 2));
+      const C t00 = A[i], t11 = A[n + i], t22 = A[2 * n + i], t01 = A[3 * n + i], t02 = A[4 * n + i],
+              t12 = A[5 * n + i];
+      // s_i = k_j a_ij ; f = -i s
+      const C s0 = mk<C>(K0 * t00.x + K1 * t01.x + K2 * t02.x, K0 * t00.y + K1 * t01.y + K2 * t02.y);
+      const C s1 = mk<C>(K0 * t01.x + K1 * t11.x + K2 * t12.x, K0 * t01.y + K1 * t11.y + K2 * t12.y);
+      const C s2 = mk<C>(K0 * t02.x + K1 * t12.x + K2 * t22.x, K0 * t02.y + K1 * t12.y + K2 * t22.y);
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2 * K2;
+      const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+      const C ks = mk<C>((K0 * s0.x + K1 * s1.x + K2 * s2.x) / den, (K0 * s0.y + K1 * s1.y + K2 * s2.y) / den);
+      const C p0 = mk<C>(s0.x - K0 * ks.x, s0.y - K0 * ks.y);
+      const C p1 = mk<C>(s1.x - K1 * ks.x, s1.y - K1 * ks.y);
+      const C p2 = mk<C>(s2.x - K2 * ks.x, s2.y - K2 * ks.y);
+      O[i] = keep ? mk<C>(p0.y, -p0.x) : czero<C>();
+      O[n + i] = keep ? mk<C>(p1.y, -p1.x) : czero<C>();
+      O[2 * n + i] = keep ? mk<C>(p2.y, -p2.x) : czero<C>();
     }
   }
 }
@@ -224,6 +245,7 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
     case EW_DIVERGENCE: hipLaunchKernelGGL((k_elementwise<Rt, EW_DIVERGENCE>), g, b, 0, s, a); return 0;
     case EW_MUL_MINUS_I: hipLaunchKernelGGL((k_elementwise<Rt, EW_MUL_MINUS_I>), g, b, 0, s, a); return 0;
     case EW_MASK_NYQUIST: hipLaunchKernelGGL((k_elementwise<Rt, EW_MASK_NYQUIST>), g, b, 0, s, a); return 0;
+    case EW_SGS_FORCE: hipLaunchKernelGGL((k_elementwise<Rt, EW_SGS_FORCE>), g, b, 0, s, a); return 0;
     default: return -1;
   }
 }

@@ -133,8 +133,10 @@ def test_batch_params_and_state_update_after_capture():
 
 def test_batch_rejects_unsupported_lengths():
     from spectraldns_amd._abi import Batch, SdnsError
-    with pytest.raises(SdnsError, match="batched kernel"):
+    with pytest.raises(SdnsError, match="kernel for line length 40"):  # no single-box kernel either
         Batch([((40, 40, 40), (TWO_PI,) * 3)], "single", "2/3-rule")
+    with pytest.raises(SdnsError, match="batched kernel"):  # 192-point lines: single-box only
+        Batch([((192, 32, 32), (TWO_PI,) * 3)], "single", "2/3-rule")
     with pytest.raises(SdnsError, match="batched kernel"):
         Batch([((48, 48, 48), (TWO_PI,) * 3)], "single", "3/2-rule")  # 72-point lines
     with pytest.raises(SdnsError):

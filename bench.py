@@ -72,7 +72,8 @@ def gpu_run(N, precision, dealias, steps, warmup, nu=0.001, dt=0.001, seed=1234)
     plan.timing_enable(False)
     passes = plan.timing_read()
     e = diag[: 2 * steps].cpu().numpy().reshape(-1, 2)
-    return elapsed, passes, e
+    variants = (plan.schedule or {}).get("variants", {})
+    return elapsed, passes, e, variants
 
 
 def ensemble_run(steps, warmup, graph=True):
@@ -111,6 +112,22 @@ def ensemble_main(args):
         "energy_range": [float(diag[:, 0].min()), float(diag[:, 0].max())],
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(N, dealias, dom, variants):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when that summary
+    covers this workload, pass and schedule variant; otherwise None."""
+    rule = {"3/2-rule": "32rule", "2/3-rule": "23rule"}[dealias]
+    path = os.path.join(ROOT, "profiles", "traffic_%dcube_%s_fp32.json" % (N[0], rule))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("pass") != dom or int(variants.get(dom, 0)) != int(t.get("variant", -1)):
+        return None
+    return {"bytes_per_launch": t["hbm_bytes_per_launch"], "kernel": t["kernel"],
+            "source": "profiles/" + t["source"]}
 
 
 def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
@@ -155,11 +172,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    elapsed, passes, diag = gpu_run(N, "single", args.dealias, args.steps, args.warmup)
+    elapsed, passes, diag, variants = gpu_run(N, "single", args.dealias, args.steps, args.warmup)
     ms_step = 1000.0 * elapsed / args.steps
     sps = args.steps / elapsed
     pb, b_step, F = pass_bytes(N, args.dealias, 8)
     dom = max((k for k in pb), key=lambda k: passes[k][0])
+    traffic = measured_traffic(N, args.dealias, dom, variants)
     dom_ms, dom_n = passes[dom]
     avg_s = dom_ms / 1000.0 / max(dom_n, 1)
     achieved = pb[dom] / avg_s / 1e9
@@ -185,7 +203,10 @@ def main():
                                   " (%d^3 padded)" % (3 * args.N // 2) if args.dealias == "3/2-rule" else ""),
                    "N": args.N, "dealias": args.dealias, "precision": "fp32"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (round(traffic["bytes_per_launch"]) if traffic else None),
+                     "algorithmic_bytes": pb[dom], "variant": int(variants.get(dom, 0)),
+                     "traffic_source": traffic["source"] if traffic else None},
         "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBps": round(b_step * sps / 1e9, 1),
                           "frac": round(b_step * sps / 1e9 / HBM_PEAK_GBS, 4)},
         "passes": per_pass,
@@ -193,7 +214,7 @@ def main():
     }
     if not args.no_fp64:
         s64 = min(args.steps, 20)
-        el64, pass64, _ = gpu_run(N, "double", args.dealias, s64, min(args.warmup, 2))
+        el64, pass64, _, _ = gpu_run(N, "double", args.dealias, s64, min(args.warmup, 2))
         pb64, b64, _ = pass_bytes(N, args.dealias, 16)
         sps64 = s64 / el64
         out["fp64"] = {"steps_per_s": round(sps64, 3), "ms_per_step": round(1000 * el64 / s64, 4),

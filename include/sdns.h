@@ -51,8 +51,10 @@ int sdns_plan_destroy(sdns_plan* plan);
 /* Schedule variant of a pass: pass ids 0..4 as in sdns_timing_read, 5 for the
  * plain transforms; variant 0 is the default schedule, 1 the alternative
  * points-per-thread factorisation where one is compiled (otherwise variant 0
- * runs).  The z pass (id 2) has variants 2 and 3 too: its LDS-assisted scheme
- * with schedule 0 / 1.  Set by the autotuner (spectraldns_amd/autotune.py). */
+ * runs).  The z pass (id 2) has variants 2 and 3 too (its LDS-assisted scheme
+ * with schedule 0 / 1) and 4 and 5 (rows staged HBM -> LDS by direct loads,
+ * schedule 0 / 1; where the rows do not fit in LDS these run variant 2 / 3).
+ * Set by the autotuner (spectraldns_amd/autotune.py). */
 int sdns_plan_set_variant(sdns_plan* plan, int pass, int variant);
 /* Rebind the plan to another HIP stream (NULL = null stream). */
 int sdns_plan_set_stream(sdns_plan* plan, void* hip_stream);

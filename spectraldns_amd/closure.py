@@ -139,3 +139,121 @@ class Closure:
                                                 float(self.in_scale), float(self.out_scale),
                                                 _ptr(diag) if diag is not None else None))
         return u_hat
+
+
+# ---------------------------------------------------------------------- experiment pipeline
+def shell_energy(plan, u_hat, nb):
+    """E(k) = 0.5 sum over the shell k - 1/2 < |K| <= k + 1/2 of w |u_hat|^2, k = 1..nb
+    (GPU shell sums, FP64), as a NumPy array."""
+    sums, _ = plan.spectrum_sums(u_hat.contiguous(), nb)
+    return 0.5 * sums.cpu().numpy()
+
+
+class ForcedRun:
+    """Forced isotropic run on one box: RK4 steps, each followed by the
+    large-scale forcing of demo/Isotropic.py:154-184 that holds
+    energy_fourier(u_hat) at its initial value on |K| <= kf.  With a closure
+    the RK4 stages add its learned Source."""
+
+    def __init__(self, N, nu, dt, u_hat=None, seed=1234, kf=2.0, precision="single", dealias="2/3-rule",
+                 k0=4.0, energy=0.5, closure=None, target=None):
+        import torch
+        from ._abi import Plan
+        from .isotropic import random_solenoidal_field
+        self.N = tuple(int(n) for n in N)
+        self.nu, self.dt, self.kf = float(nu), float(dt), float(kf)
+        self.plan = Plan(self.N, (2 * np.pi,) * 3, precision, dealias)
+        cd = torch.complex64 if precision == "single" else torch.complex128
+        if u_hat is None:
+            u_hat = torch.from_numpy(random_solenoidal_field(self.N, seed=seed, k0=k0, energy=energy)).to(cd).cuda()
+        self.u_hat = u_hat.contiguous()
+        self.plan.mask_nyquist(self.u_hat)
+        self.closure = closure
+        self._e = torch.zeros(2, dtype=torch.float64, device="cuda")
+        self.target = self.energy_fourier() if target is None else float(target)
+        self.steps_done = 0
+
+    def energy_fourier(self):
+        self.plan.energy(self.u_hat, self.nu, self._e)
+        return 2.0 * float(self._e[0].item())
+
+    def diagnostics(self):
+        """(kinetic energy, resolved dissipation) in FP64."""
+        self.plan.energy(self.u_hat, self.nu, self._e)
+        e = self._e.cpu().numpy()
+        return float(e[0]), float(e[1])
+
+    def step(self, n=1):
+        for _ in range(n):
+            if self.closure is None:
+                self.plan.rk4_steps(self.u_hat, [self.dt], self.nu)
+            else:
+                self.closure.rk4_steps(self.plan, self.u_hat, [self.dt], self.nu)
+            self.plan.force(self.u_hat, self.target, self.kf)
+        self.steps_done += n
+
+    def physical(self):
+        import torch
+        u = torch.empty((3,) + self.N, dtype=self.plan.real_dtype, device="cuda")
+        return self.plan.backward(self.u_hat, u)
+
+    def filtered(self, r):
+        """Box-filtered velocity and subgrid stress (physical, FP32) of the current state."""
+        return self.plan.box_filter(self.physical(), r)
+
+
+def coarse_state(plan_c, ubar):
+    """Spectral state of the coarse plan from a filtered physical velocity, Nyquist masked."""
+    import torch
+    cd = torch.complex64 if plan_c.precision == "single" else torch.complex128
+    u_hat = torch.empty((3,) + plan_c.spectral_shape, dtype=cd, device="cuda")
+    plan_c.forward(ubar.to(plan_c.real_dtype).contiguous(), u_hat)
+    plan_c.mask_nyquist(u_hat)
+    return u_hat
+
+
+def scales(pairs):
+    """(in_scale, out_scale): 1 / rms of the filtered velocity, rms of tau."""
+    import torch
+    su = torch.sqrt(torch.stack([torch.mean(u.double() ** 2) for u, _ in pairs]).mean()).item()
+    st = torch.sqrt(torch.stack([torch.mean(t.double() ** 2) for _, t in pairs]).mean()).item()
+    return 1.0 / su, st
+
+
+def train(closure, pairs, iters, lr0=2e-3, lr1=1e-4, log_every=100, val=None):
+    """Adam over the pairs (cycled), learning rate decayed geometrically from
+    lr0 to lr1.  Returns [(iteration, mean training loss since the last log,
+    validation loss or None)]."""
+    import torch
+    hist = []
+    acc = torch.zeros(1, dtype=torch.float64, device="cuda")
+    cnt = 0
+    for it in range(iters):
+        u, t = pairs[it % len(pairs)]
+        acc += closure.train_step(u, t)
+        cnt += 1
+        lr = lr0 * (lr1 / lr0) ** (it / max(iters - 1, 1))
+        closure.adam(lr)
+        if (it + 1) % log_every == 0 or it == iters - 1:
+            vl = None
+            if val is not None:
+                vl = float(np.mean([validation_loss(closure, u2, t2) for u2, t2 in val]))
+            hist.append((it + 1, float(acc.item()) / cnt, vl))
+            acc.zero_()
+            cnt = 0
+    return hist
+
+
+def validation_loss(closure, u, t):
+    """Normalised loss of one pair without touching the gradients: mean over
+    points and components of ((tau_pred - tau) / out_scale)^2."""
+    tau = closure.forward(u)
+    return float(torch_mean_sq((tau - t) / closure.out_scale))
+
+
+def zero_model_loss(closure, t):
+    return float(torch_mean_sq(t / closure.out_scale))
+
+
+def torch_mean_sq(x):
+    return (x.double() ** 2).mean().item()

@@ -113,7 +113,7 @@ class Plan:
         self.precision = precision
         self.dealias = dealias
         self.dealias_mode = int(dealias_mode)
-        self.mask_nyquist = bool(mask_nyquist)
+        self.nyquist_masked = bool(mask_nyquist)
         self.real_dtype = torch.float32 if precision == "single" else torch.float64
         self.complex_dtype = torch.complex64 if precision == "single" else torch.complex128
         self.device = torch.device("cuda", torch.cuda.current_device())
@@ -123,7 +123,7 @@ class Plan:
         stream = torch.cuda.current_stream().cuda_stream
         _check(lib.sdns_plan_create(ctypes.byref(h), Na, La,
                                     SDNS_SINGLE if precision == "single" else SDNS_DOUBLE,
-                                    DEALIAS[dealias], int(self.mask_nyquist), self.dealias_mode,
+                                    DEALIAS[dealias], int(self.nyquist_masked), self.dealias_mode,
                                     _P(stream)))
         self._h = h
         self._lib = lib

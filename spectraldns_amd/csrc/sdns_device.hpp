@@ -218,16 +218,33 @@ constexpr int zline_E(int n, int v = 0) {
 constexpr int strided_variants(int M) { return strided_E(M, 1) != strided_E(M, 0) ? 2 : 1; }
 constexpr int zline_variants(int n) { return zline_E(n, 1) != zline_E(n, 0) ? 2 : 1; }
 
+// Barrier of an LDS exchange.  WAVE: every thread that exchanges data is in
+// the calling wave; LDS operations of one wave complete in issue order, so a
+// wave-level barrier (no s_barrier) and wave-scope fences that stop the
+// compiler from moving LDS accesses across it suffice.  Otherwise the block
+// barrier.
+template <bool WAVE> __device__ __forceinline__ void lds_barrier() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // tw[m] = exp(-2 pi i m / M), m in [0, M): the forward twiddle table of the
-// line length; the inverse direction uses its conjugate.
-template <typename Rt, int M, int E, int DIR> struct LineFFT {
+// line length; the inverse direction uses its conjugate.  WAVE: the P threads
+// of a line are in one wave (exchanges need no block barrier).
+template <typename Rt, int M, int E, int DIR, bool WAVE = false> struct LineFFT {
   using C = cx<Rt>;
   static constexpr int P = M / E;
   static constexpr int NP = num_passes(M, E);
   static_assert(E > 0 && M % E == 0 && NP > 0, "unsupported line length");
 
   // lds: this line's exchange area, element p at lds[p * es].  Every thread
-  // of the block must call run() (it contains __syncthreads when NP > 1).
+  // of the block (of the wave, if WAVE) must call run() (it contains barriers
+  // when NP > 1).
   __device__ static __forceinline__ void run(C (&v)[E], int t, C* lds, int es, const C* __restrict__ tw) {
     pass<0, 1>(v, t, lds, es, tw);
   }
@@ -262,10 +279,10 @@ template <typename Rt, int M, int E, int DIR> struct LineFFT {
 #pragma unroll
         for (int k = 0; k < R; ++k) lds[(base + k * NS) * es] = v[b + k * NB];
       }
-      __syncthreads();
+      lds_barrier<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = lds[(t + P * e) * es];
-      __syncthreads();
+      lds_barrier<WAVE>();
       pass<PASS + 1, NS * R>(v, t, lds, es, tw);
     }
   }

@@ -275,14 +275,14 @@ __device__ __forceinline__ void c2r_pre_lds(cx<Rt> (&z)[E], const cx<Rt>* my, co
 // Post-process of the forward half-length FFT held in z (natural positions
 // t + P e); `my` is this line's LDS area.  Writes kept bins k < nk (and the
 // Nyquist bin n when nk > n).
-template <typename Rt, int n, int E>
+template <typename Rt, int n, int E, bool WAVE = false>
 __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __restrict__ out, int nk, bool valid,
                                          const cx<Rt>* __restrict__ rtw, int t, Rt scale) {
   using C = cx<Rt>;
   constexpr int P = n / E;
 #pragma unroll
   for (int e = 0; e < E; ++e) my[t + P * e] = z[e];
-  __syncthreads();
+  lds_barrier<WAVE>();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + P * e;
@@ -295,7 +295,7 @@ __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __r
     if (valid && k < nk) out[k] = cscale(X, scale);
     if (valid && k == 0 && nk > n) out[n] = mk<C>((A.x - A.y) * scale, Rt(0));
   }
-  __syncthreads();
+  lds_barrier<WAVE>();
 }
 
 // Staged load of one spectral line: the P threads of a line read bins
@@ -327,12 +327,14 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
-  using Fi = LineFFT<Rt, n, E, +1>;
-  using Ff = LineFFT<Rt, n, E, -1>;
+  constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
+  using Fi = LineFFT<Rt, n, E, +1, WL>;
+  using Ff = LineFFT<Rt, n, E, -1, WL>;
   __shared__ C lds[LPB * (n + ZPAD)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
+  __syncthreads();
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
@@ -346,9 +348,9 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     st.store(my, t);
-    __syncthreads();
+    lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(ph[f], my, rtw, t);
-    __syncthreads();
+    lds_barrier<WL>();
     if (f < 5) st.load(in + (f + 1) * ifield, a.nkz, valid, t);  // next field in flight during the FFT
     Fi::run(ph[f], t, my, 1, tw);
   }
@@ -367,7 +369,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     Ff::run(ph[i], t, my, 1, tw);
-    r2c_post<Rt, n, E>(ph[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+    r2c_post<Rt, n, E, WL>(ph[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
   }
 }
 
@@ -394,8 +396,10 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
                                          const cx<Rt>* rtw) {
   using C = cx<Rt>;
   constexpr int P = n / E, SL = n + ZPAD;
-  using Fi = LineFFT<Rt, n, E, +1>;
-  using Ff = LineFFT<Rt, n, E, -1>;
+  constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
+  using Fi = LineFFT<Rt, n, E, +1, WL>;
+  using Ff = LineFFT<Rt, n, E, -1, WL>;
+  __syncthreads();  // twiddle tables written by the caller
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = line0 + l;
   const bool valid = line < a.L;
@@ -412,9 +416,9 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     st.store(s0, t);
-    __syncthreads();
+    lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(U[f], s0, rtw, t);
-    __syncthreads();
+    lds_barrier<WL>();
     st.load(in + (f + 1) * ifield, a.nkz, valid, t);
     Fi::run(U[f], t, s0, 1, tw);
   }
@@ -423,9 +427,9 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   for (int f = 0; f < 2; ++f) {
     C* sl = f == 0 ? s1 : s2;
     st.store(sl, t);
-    __syncthreads();
+    lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(W, sl, rtw, t);
-    __syncthreads();
+    lds_barrier<WL>();
     st.load(in + (4 + f) * ifield, a.nkz, valid, t);
     Fi::run(W, t, sl, 1, tw);
 #pragma unroll
@@ -433,11 +437,11 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   }
   // w2 -> registers (slot 0)
   st.store(s0, t);
-  __syncthreads();
+  lds_barrier<WL>();
   c2r_pre_lds<Rt, n, E>(W, s0, rtw, t);
-  __syncthreads();
+  lds_barrier<WL>();
   Fi::run(W, t, s0, 1, tw);
-  __syncthreads();
+  lds_barrier<WL>();
   // u x w on the packed points; the products overwrite the velocity registers
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -448,14 +452,14 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
     U[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
     U[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
   }
-  __syncthreads();
+  lds_barrier<WL>();
   C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
   const long ofield = (long)a.L * a.nkzop;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     C* sl = i == 0 ? s0 : (i == 1 ? s1 : s2);
     Ff::run(U[i], t, sl, 1, tw);
-    r2c_post<Rt, n, E>(U[i], sl, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+    r2c_post<Rt, n, E, WL>(U[i], sl, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
   }
 }
 
@@ -470,17 +474,153 @@ __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + s
   p3b_body<Rt, n, E, LPB>(a, blockIdx.x * LPB, lds, tw, rtw);
 }
 
+// ------------------------------------------------------------------ z pass, LDS-DMA scheme
+// Scheme C: at the start of a block the six spectral z-line rows of all its
+// LPB lines are copied HBM -> LDS by global_load_lds_dwordx4 (no registers
+// involved, so every row of the block is in flight at once; a block's lines
+// are consecutive rows, so each field is one contiguous chunk).  Then, per
+// line, the six C2R transforms run from the staged rows, w0 and w1 are parked
+// in LDS over the consumed velocity rows, the cross product is formed in
+// registers and the three R2C transforms go back to HBM.  Registers hold 4E
+// complex values and no prefetch.  LDS (dynamic, bytes from zc_lds_bytes):
+//   [6][LPB][nkzp] staging | [LPB][n+ZPAD] exchange | w0/w1 slots if they do
+//   not fit over the staged u rows | tw[n] | rtw[n].
+
+// Copy `bytes` (a multiple of 16) from global src to LDS dst: 16 bytes per
+// lane, a wave fills 1 KB of LDS per instruction.  blockDim.x % 64 == 0.
+template <typename C>
+__device__ __forceinline__ void lds_dma_copy(C* dst, const C* src, int bytes) {
+  typedef __attribute__((address_space(3))) void lds_void;
+  typedef __attribute__((address_space(1))) void glb_void;
+  const int units = bytes >> 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int u0 = wave * 64; u0 < units; u0 += nw * 64) {
+    const int u = u0 + lane;
+    if (u < units)
+      __builtin_amdgcn_global_load_lds((glb_void*)((const char*)src + (long)u * 16),
+                                       (lds_void*)((char*)dst + (long)u0 * 16), 16, 0, 0);
+  }
+}
+
+// Z[t + P e] of a staged spectral row my[0..nk) (entries at and above nk are zero).
+template <typename Rt, int n, int E>
+__device__ __forceinline__ void c2r_pre_row(cx<Rt> (&z)[E], const cx<Rt>* my, int nk, const cx<Rt>* rtw, int t) {
+  using C = cx<Rt>;
+  constexpr int P = n / E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    C xa = p < nk ? my[p] : czero<C>();
+    C xb = n - p < nk ? my[n - p] : czero<C>();
+    if (p == 0) { xa.y = 0; xb.y = 0; }
+    const C ev = mk<C>(xa.x + xb.x, xa.y - xb.y);
+    const C d = mk<C>(xa.x - xb.x, xa.y + xb.y);
+    const C od = cmulc(d, rtw[p]);
+    z[e] = mk<C>(ev.x - od.y, ev.y + od.x);
+  }
+}
+
+struct ZcLayout {
+  int stage, xch, om, tw, rtw, total;  // offsets / size in complex elements
+};
+__host__ __device__ inline ZcLayout zc_layout(int n, int lpb, int nkzp) {
+  ZcLayout z;
+  z.stage = 0;
+  z.xch = 6 * lpb * nkzp;
+  const int slot = n + ZPAD;
+  const bool fits = 2 * slot <= 3 * nkzp;  // w0, w1 over the staged u rows of the same block
+  z.om = fits ? 0 : z.xch + lpb * slot;
+  z.tw = z.xch + lpb * slot + (fits ? 0 : 2 * lpb * slot);
+  z.rtw = z.tw + n;
+  z.total = z.rtw + n;
+  return z;
+}
+
+// Occupancy target 2: a block's staged rows take up to 64 KB of LDS, so at most
+// two blocks share a CU whatever the register budget.
+template <typename Rt, int n, int E, int LPB>
+__global__ void __launch_bounds__(LPB * (n / E), 2) k_p3c(P3Args<Rt> a) {
+  using C = cx<Rt>;
+  constexpr int P = n / E, SL = n + ZPAD;
+  static_assert((LPB * P) % 64 == 0, "whole waves");
+  constexpr bool WL = 64 % P == 0;
+  using Fi = LineFFT<Rt, n, E, +1, WL>;
+  using Ff = LineFFT<Rt, n, E, -1, WL>;
+  extern __shared__ __align__(16) unsigned char zc_smem[];
+  C* sm = (C*)zc_smem;
+  const ZcLayout Z = zc_layout(n, LPB, a.nkzp);
+  C* st = sm + Z.stage;
+  C* tw = sm + Z.tw;
+  C* rtw = sm + Z.rtw;
+  const int line0 = blockIdx.x * LPB;
+  const int nl = min(LPB, a.L - line0);
+  const long ifield = (long)a.L * a.nkzp;
+  const int chunk = nl * a.nkzp;  // complex elements per field
+  const C* __restrict__ in = (const C*)a.w2 + (long)line0 * a.nkzp;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) lds_dma_copy(st + f * LPB * a.nkzp, in + f * ifield, chunk * (int)sizeof(C));
+  table_to_lds(tw, (const C*)a.tw, n);
+  table_to_lds(rtw, (const C*)a.rtw, n);
+  __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  const int t = threadIdx.x % P, l = threadIdx.x / P;
+  const int line = line0 + l;
+  const bool valid = line < a.L;
+  const long li = valid ? line : 0;
+  C* xl = sm + Z.xch + l * SL;
+  C* o0 = sm + Z.om + (2 * l) * SL;
+  C* o1 = o0 + SL;
+  auto row = [&](int f) { return (const C*)(st + (f * LPB + l) * a.nkzp); };
+  C U[3][E], W[E];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    c2r_pre_row<Rt, n, E>(U[f], row(f), a.nkz, rtw, t);
+    Fi::run(U[f], t, xl, 1, tw);
+  }
+  __syncthreads();  // every line's u rows are consumed before w0/w1 may overwrite them
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    c2r_pre_row<Rt, n, E>(W, row(3 + f), a.nkz, rtw, t);
+    Fi::run(W, t, xl, 1, tw);
+    C* om = f == 0 ? o0 : o1;
+#pragma unroll
+    for (int e = 0; e < E; ++e) om[t + P * e] = W[e];
+  }
+  c2r_pre_row<Rt, n, E>(W, row(5), a.nkz, rtw, t);
+  Fi::run(W, t, xl, 1, tw);
+  lds_barrier<WL>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    const C u0 = U[0][e], u1 = U[1][e], u2 = U[2][e];
+    const C w0 = o0[p], w1 = o1[p], w2 = W[e];
+    U[0][e] = mk<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+    U[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+    U[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+  }
+  lds_barrier<WL>();
+  C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
+  const long ofield = (long)a.L * a.nkzop;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Ff::run(U[i], t, xl, 1, tw);
+    r2c_post<Rt, n, E, WL>(U[i], xl, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+  }
+}
+
 // ------------------------------------------------------------------ plain z transforms
 template <typename Rt, int n, int E>
 __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_c2r(ZArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
-  using Fi = LineFFT<Rt, n, E, +1>;
+  constexpr bool WL = 64 % P == 0;
+  using Fi = LineFFT<Rt, n, E, +1, WL>;
   __shared__ C lds[LPB * (n + ZPAD)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
+  __syncthreads();
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
@@ -490,10 +630,10 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   LineStage<Rt, n, P> st;
   st.load(in, a.nk, valid, t);
   st.store(my, t);
-  __syncthreads();
+  lds_barrier<WL>();
   C z[E];
   c2r_pre_lds<Rt, n, E>(z, my, rtw, t);
-  __syncthreads();
+  lds_barrier<WL>();
   Fi::run(z, t, my, 1, tw);
   if (valid) {
     C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;  // real line viewed as n complex pairs
@@ -507,11 +647,13 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
-  using Ff = LineFFT<Rt, n, E, -1>;
+  constexpr bool WL = 64 % P == 0;
+  using Ff = LineFFT<Rt, n, E, -1, WL>;
   __shared__ C lds[LPB * (n + ZPAD)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
+  __syncthreads();
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
@@ -521,10 +663,10 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   C z[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) z[e] = valid ? in[t + P * e] : czero<C>();
-  __syncthreads();
+  lds_barrier<WL>();
   Ff::run(z, t, my, 1, tw);
   C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;
-  r2c_post<Rt, n, E>(z, my, out, a.nk, valid, rtw, t, a.scale);
+  r2c_post<Rt, n, E, WL>(z, my, out, a.nk, valid, rtw, t, a.scale);
 }
 
 // ------------------------------------------------------------------ generic strided

@@ -25,6 +25,24 @@ template <typename Rt, int n, int E> void p3b_go(const P3Args<Rt>& a, hipStream_
   using Cfg = ZBCfg<Rt, n, E>;
   hipLaunchKernelGGL((k_p3b<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
 }
+// Scheme C (LDS-DMA staging) with LPB lines per block, if its block is whole
+// waves and its LDS fits in 64 KB; false otherwise.
+template <typename Rt, int n, int E, int LPB> bool p3c_try(const P3Args<Rt>& a, hipStream_t s) {
+  constexpr int TH = LPB * (n / E);
+  if constexpr (n < 32 || TH % 64 != 0 || TH > 512) {
+    return false;
+  } else {
+    const size_t bytes = (size_t)zc_layout(n, LPB, a.nkzp).total * sizeof(cx<Rt>);
+    if (bytes > 65536) return false;
+    hipLaunchKernelGGL((k_p3c<Rt, n, E, LPB>), dim3((a.L + LPB - 1) / LPB), dim3(TH), bytes, s, a);
+    return true;
+  }
+}
+template <typename Rt, int n, int E> void p3c_go(const P3Args<Rt>& a, hipStream_t s) {
+  if (p3c_try<Rt, n, E, 8>(a, s)) return;
+  if (p3c_try<Rt, n, E, 4>(a, s)) return;
+  p3b_go<Rt, n, E>(a, s);  // rows too long for the LDS budget: LDS-assisted scheme
+}
 template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
@@ -89,12 +107,15 @@ template <typename Rt> int launch_p2(int M, int var, const P2Args<Rt>& a, hipStr
 }
 
 // P3 variants: 0 and 1 = all six lines in registers with schedule 0 / 1;
-// 2 and 3 = the LDS-assisted scheme with schedule 0 / 1.
+// 2 and 3 = the LDS-assisted scheme with schedule 0 / 1; 4 and 5 = the
+// LDS-DMA staging scheme with schedule 0 / 1.
 template <typename Rt> int launch_p3(int n, int var, const P3Args<Rt>& a, hipStream_t s) {
   switch (n) {
 #define SDNS_X(m)                                                        \
   case m:                                                                \
-    if (var == 3) p3b_go<Rt, m, zline_E(m, 1)>(a, s);                    \
+    if (var == 5) p3c_go<Rt, m, zline_E(m, 1)>(a, s);                    \
+    else if (var == 4) p3c_go<Rt, m, zline_E(m, 0)>(a, s);               \
+    else if (var == 3) p3b_go<Rt, m, zline_E(m, 1)>(a, s);               \
     else if (var == 2) p3b_go<Rt, m, zline_E(m, 0)>(a, s);               \
     else if (var == 1) p3_go<Rt, m, zline_E(m, 1)>(a, s);                \
     else p3_go<Rt, m, zline_E(m, 0)>(a, s);                              \

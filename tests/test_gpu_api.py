@@ -83,3 +83,27 @@ def test_context_keys_and_accessors():
     assert np.max(abs(P.cpu().numpy() - g.backward(-1j * P_ref))) < 1e-13
     div = solver.get_divergence(**c)
     assert float(div.abs().max()) < 1e-14
+
+
+def test_compute_rhs_granular_path():
+    """ComputeRHS with a user-supplied conv (not the fused Vortex one) runs
+    conv -> Nyquist mask -> add_pressure_diffusion -> Source step by step
+    (NS.py:219-261) and agrees with the oracle."""
+    from spectraldns_amd import config, get_solver
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    from oracle.ns import Grid
+    config.update({"nu": 0.01, "dt": 0.01, "T": 0.02, "dealias": "2/3-rule", "precision": "double"})
+    solver = get_solver(parse_args=["--M", "4", "4", "4", "NS"])
+    c = solver.get_context()
+    u = random_solenoidal_field((16, 16, 16), seed=5, k0=3.0, energy=0.5)
+    c.U_hat.copy_(torch.from_numpy(u))
+    base = solver.getConvection("Vortex")
+
+    def conv(rhs, u_hat, work, Tp, VTp, K, dummy):
+        return base(rhs, u_hat, work, Tp, VTp, K, dummy)
+
+    solver.conv = conv
+    rhs = solver.ComputeRHS(c.dU, c.U_hat, solver, **c)
+    g = Grid((16, 16, 16))
+    ref = g.compute_rhs(u, 0.01)
+    assert np.max(abs(rhs.cpu().numpy() - ref)) < 1e-13 * np.max(abs(ref))

@@ -222,6 +222,29 @@ def test_alternative_schedules_fp64(N, rule):
     assert rel(ud.cpu().numpy(), g.rk4_step(g.rk4_step(u, 0.01, nu), 0.01, nu)) < 1e-12
 
 
+@pytest.mark.parametrize("N,rule", [((64, 48, 128), "2/3-rule"), ((96, 64, 128), "3/2-rule"),
+                                    ((32, 32, 256), "3/2-rule"), ((64, 64, 96), "None")])
+def test_z_pass_variants_fp64(N, rule):
+    """Every storage scheme / schedule of the z pass (variants 0..5) keeps FP64
+    parity with the oracle; variants 4/5 (rows staged by direct HBM -> LDS
+    loads) equal variants 2/3 bit for bit (same arithmetic)."""
+    L = (TWO_PI, 4.0, 3.0)
+    nu = 0.01
+    g = Grid(N, L, "double", rule)
+    u = field(N, L, seed=23)
+    ref = g.compute_rhs(u, nu)
+    out = {}
+    for var in range(6):
+        p = plan_of(N, L, "double", rule)
+        p.set_variant(2, var)
+        ud = to_gpu(u, torch.complex128)
+        rhs = torch.empty_like(ud)
+        p.compute_rhs(ud, rhs, nu)
+        out[var] = rhs.cpu().numpy()
+        assert rel(out[var], ref) < 1e-13, var
+    assert np.array_equal(out[2], out[4]) and np.array_equal(out[3], out[5])
+
+
 def test_compensated_fp32_accumulation():
     """Many small steps: plain FP32 accumulation of u_hat drifts by about one
     rounding per step, the compensated (hi, lo) pair stays at FP32 resolution

@@ -213,10 +213,14 @@ int sdns_adam_step(float* params, const float* grad, float* m, float* v, size_t 
  * fields) = bar(u_i u_j) - ubar_i ubar_j with FP64 sums. */
 int sdns_box_filter(sdns_plan* plan, const void* u_phys, int r, void* ubar, void* tau);
 /* sdns_rk4_steps on a single-precision plan of the closure's grid with the
- * closure's Source added at every stage (the stage state's velocity is
- * transformed back, evaluated by the network, tau transformed forward). */
+ * closure's Source added to every stage's right-hand side.  per_stage = 0:
+ * the Source is evaluated once per step from the step's initial state and held
+ * through the four stages, as the reference holds Source during a step
+ * (it is set between steps by the update hook); per_stage = 1: evaluated from
+ * every stage state.  Evaluation: the state's velocity is transformed back,
+ * passed through the network, tau transformed forward. */
 int sdns_closure_rk4_steps(sdns_closure* closure, sdns_plan* plan, void* u_hat, const double* dt_seq, int nsteps,
-                           double nu, float in_scale, float out_scale, double* diag);
+                           double nu, float in_scale, float out_scale, int per_stage, double* diag);
 
 /* Human-readable message of the last failing call on this thread. */
 const char* sdns_last_error(void);

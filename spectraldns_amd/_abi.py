@@ -63,7 +63,7 @@ _SIGS = {
                                       ctypes.c_float, ctypes.c_float, ctypes.c_int, _P]),
     "sdns_box_filter": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, _P]),
     "sdns_closure_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
-                                              ctypes.c_double, ctypes.c_float, ctypes.c_float, _P]),
+                                              ctypes.c_double, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P]),
     "sdns_last_error": (ctypes.c_char_p, []),
 }
 

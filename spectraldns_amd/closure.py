@@ -123,8 +123,9 @@ class Closure:
                                         _P(torch.cuda.current_stream().cuda_stream)))
         _check(self._lib.sdns_closure_set_params(self._h, _ptr(self.params)))
 
-    def rk4_steps(self, plan, u_hat, dts, nu, diag=None):
-        """RK4 steps of the coarse plan with the learned Source at every stage."""
+    def rk4_steps(self, plan, u_hat, dts, nu, diag=None, per_stage=False):
+        """RK4 steps of the coarse plan with the learned Source, evaluated once
+        per step (held through the stages) or, with per_stage, at every stage."""
         import torch
         if plan.N != self.N or plan.precision != "single":
             raise SdnsError("the closure runs on a single-precision plan of its own grid")
@@ -136,7 +137,7 @@ class Closure:
         plan._sync_stream()
         self._stream()
         _check(self._lib.sdns_closure_rk4_steps(self._h, plan._h, _ptr(u_hat), arr, n, float(nu),
-                                                float(self.in_scale), float(self.out_scale),
+                                                float(self.in_scale), float(self.out_scale), int(bool(per_stage)),
                                                 _ptr(diag) if diag is not None else None))
         return u_hat
 
@@ -156,19 +157,20 @@ class ForcedRun:
     the RK4 stages add its learned Source."""
 
     def __init__(self, N, nu, dt, u_hat=None, seed=1234, kf=2.0, precision="single", dealias="2/3-rule",
-                 k0=4.0, energy=0.5, closure=None, target=None):
+                 k0=4.0, energy=0.5, closure=None, target=None, per_stage=False, dealias_mode=0):
         import torch
         from ._abi import Plan
         from .isotropic import random_solenoidal_field
         self.N = tuple(int(n) for n in N)
         self.nu, self.dt, self.kf = float(nu), float(dt), float(kf)
-        self.plan = Plan(self.N, (2 * np.pi,) * 3, precision, dealias)
+        self.plan = Plan(self.N, (2 * np.pi,) * 3, precision, dealias, True, dealias_mode)
         cd = torch.complex64 if precision == "single" else torch.complex128
         if u_hat is None:
             u_hat = torch.from_numpy(random_solenoidal_field(self.N, seed=seed, k0=k0, energy=energy)).to(cd).cuda()
         self.u_hat = u_hat.contiguous()
         self.plan.mask_nyquist(self.u_hat)
         self.closure = closure
+        self.per_stage = per_stage
         self._e = torch.zeros(2, dtype=torch.float64, device="cuda")
         self.target = self.energy_fourier() if target is None else float(target)
         self.steps_done = 0
@@ -188,7 +190,7 @@ class ForcedRun:
             if self.closure is None:
                 self.plan.rk4_steps(self.u_hat, [self.dt], self.nu)
             else:
-                self.closure.rk4_steps(self.plan, self.u_hat, [self.dt], self.nu)
+                self.closure.rk4_steps(self.plan, self.u_hat, [self.dt], self.nu, per_stage=self.per_stage)
             self.plan.force(self.u_hat, self.target, self.kf)
         self.steps_done += n
 
@@ -202,14 +204,113 @@ class ForcedRun:
         return self.plan.box_filter(self.physical(), r)
 
 
-def coarse_state(plan_c, ubar):
-    """Spectral state of the coarse plan from a filtered physical velocity, Nyquist masked."""
+def keep_mask(plan, kc):
+    """Boolean spectral mask |k_i| <= kc on every axis (tensor product, as the
+    solver's 2/3-rule keep sets), Nyquist planes excluded."""
+    import torch
+    N = plan.N
+    kx = torch.fft.fftfreq(N[0], 1.0 / N[0], device="cuda").abs()
+    ky = torch.fft.fftfreq(N[1], 1.0 / N[1], device="cuda").abs()
+    kz = torch.arange(N[2] // 2 + 1, device="cuda", dtype=kx.dtype)
+    m = (kx[:, None, None] <= kc) & (ky[None, :, None] <= kc) & (kz[None, None, :] <= kc)
+    m &= (kx[:, None, None] < N[0] // 2) & (ky[None, :, None] < N[1] // 2) & (kz[None, None, :] < N[2] // 2)
+    return m
+
+
+def coarse_state(plan_c, ubar, kc=None):
+    """Spectral state of the coarse plan from a filtered physical velocity:
+    Nyquist masked and, with kc, band-limited to |k_i| <= kc."""
     import torch
     cd = torch.complex64 if plan_c.precision == "single" else torch.complex128
     u_hat = torch.empty((3,) + plan_c.spectral_shape, dtype=cd, device="cuda")
     plan_c.forward(ubar.to(plan_c.real_dtype).contiguous(), u_hat)
     plan_c.mask_nyquist(u_hat)
+    if kc is not None:
+        u_hat *= keep_mask(plan_c, kc)
     return u_hat
+
+
+def band_limit(plan_c, f, kc):
+    """Physical fields f (c, n0, n1, n2) with every mode |k_i| > kc removed
+    (forward transform, mask, backward transform on the coarse plan)."""
+    import torch
+    out = torch.empty_like(f)
+    cd = torch.complex64 if plan_c.precision == "single" else torch.complex128
+    m = keep_mask(plan_c, kc)
+    for c0 in range(0, f.shape[0], 3):
+        blk = f[c0:c0 + 3].contiguous()
+        h = torch.empty((blk.shape[0],) + plan_c.spectral_shape, dtype=cd, device="cuda")
+        plan_c.forward(blk, h)
+        h *= m
+        o = torch.empty_like(blk)
+        plan_c.backward(h, o)
+        out[c0:c0 + 3] = o
+    return out
+
+
+def filtered_pair(run, r, plan_c, kc):
+    """Training pair for the composed filter G = (spectral cutoff kc) o (box r):
+    bar u = G u and tau_ij = G(u_i u_j) - bar u_i bar u_j (deviatoric), from the
+    box filter's block means of u and u_i u_j on the coarse grid."""
+    import torch
+    ub, tau = run.filtered(r)
+    ub = ub.float().contiguous()
+    T = tau.float().clone()
+    for m, (i, j) in enumerate(_PAIRS):
+        T[m] += ub[i] * ub[j]  # box means of the products
+    ubl = band_limit(plan_c, ub, kc)
+    Tl = band_limit(plan_c, T, kc)
+    for m, (i, j) in enumerate(_PAIRS):
+        Tl[m] -= ubl[i] * ubl[j]
+    return ubl.contiguous(), deviatoric(Tl).contiguous()
+
+
+def deviatoric(tau):
+    """tau_ij - delta_ij tau_kk / 3 (in place on the (6, ...) xx, yy, zz, xy, xz, yz
+    fields).  The isotropic part only adds a gradient to the Source, which the
+    projection removes, so the network is trained on the deviatoric part."""
+    tr = (tau[0] + tau[1] + tau[2]) / 3.0
+    tau[:3] -= tr
+    return tau
+
+
+# component pairs of the six stress fields
+_PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+_INDEX = {p: m for m, p in enumerate(_PAIRS)}
+_INDEX.update({(j, i): m for (i, j), m in _INDEX.items()})
+
+
+def symmetry(u, tau, perm, signs):
+    """Image of a training pair under the cube symmetry x'_i = s_i x_{perm[i]}
+    (periodic; NS is invariant under rotations and reflections):
+    u'_i(x') = s_i u_{perm[i]}(x), tau'_ij(x') = s_i s_j tau_{perm[i] perm[j]}(x)."""
+    import torch
+    u2 = u[list(perm)].permute(0, *[1 + p for p in perm])
+    t2 = torch.stack([tau[_INDEX[(perm[i], perm[j])]] for i, j in _PAIRS]).permute(0, *[1 + p for p in perm])
+    for ax in range(3):
+        if signs[ax] < 0:
+            # x -> -x on the periodic lattice: index i -> (-i) mod N
+            u2 = torch.roll(torch.flip(u2, dims=(1 + ax,)), 1, dims=1 + ax)
+            t2 = torch.roll(torch.flip(t2, dims=(1 + ax,)), 1, dims=1 + ax)
+    sg = torch.tensor(signs, dtype=u.dtype, device=u.device)
+    u2 = u2 * sg.view(3, 1, 1, 1)
+    st = torch.tensor([signs[i] * signs[j] for i, j in _PAIRS], dtype=tau.dtype, device=tau.device)
+    t2 = t2 * st.view(6, 1, 1, 1)
+    return u2.contiguous(), t2.contiguous()
+
+
+def augment(pairs, count, seed=0):
+    """`count` random cube-symmetry images of randomly chosen pairs."""
+    import itertools
+    rng = np.random.Generator(np.random.PCG64(seed))
+    perms = list(itertools.permutations(range(3)))
+    out = []
+    for _ in range(count):
+        u, t = pairs[int(rng.integers(len(pairs)))]
+        perm = perms[int(rng.integers(6))]
+        signs = tuple(int(x) for x in rng.choice([-1, 1], 3))
+        out.append(symmetry(u, t, perm, signs))
+    return out
 
 
 def scales(pairs):

@@ -432,9 +432,12 @@ int closure_source(sdns_closure* c, sdns_plan* p, const void* u, float in_scale,
   return 0;
 }
 
+// With a closure: its Source is evaluated from the step's initial state and
+// held through the four stages (as the reference's Source, set between steps),
+// or, with per_stage, from every stage state.
 template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const double* dts, int nsteps, double nu,
                                       double* diag, void* ph, sdns_closure* cl = nullptr, float in_scale = 1.f,
-                                      float out_scale = 1.f) {
+                                      float out_scale = 1.f, int per_stage = 0) {
   // a, b and dt in the working precision (integrators.py:185-186, config.py:123-127)
   const Rt a[4] = {Rt(1. / 6.), Rt(1. / 3.), Rt(1. / 3.), Rt(1. / 6.)};
   const Rt b[3] = {Rt(0.5), Rt(0.5), Rt(1.)};
@@ -444,7 +447,7 @@ template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const dou
     for (int rk = 0; rk < 4; ++rk) {
       int rc;
       const void* stage_in = rk == 0 ? u : (const void*)p->stB;
-      if (cl && (rc = closure_source(cl, p, stage_in, in_scale, out_scale))) return rc;
+      if (cl && (per_stage || rk == 0) && (rc = closure_source(cl, p, stage_in, in_scale, out_scale))) return rc;
       if ((rc = conv_passes<Rt>(p, stage_in))) return rc;
       P5Args<Rt> pa = p5_args<Rt>(p, nu);
       pa.A = u; pa.B = p->stB; pa.Cacc = p->stC;
@@ -1257,7 +1260,7 @@ int sdns_box_filter(sdns_plan* p, const void* u_phys, int r, void* ubar, void* t
 }
 
 int sdns_closure_rk4_steps(sdns_closure* c, sdns_plan* p, void* u, const double* dts, int nsteps, double nu,
-                           float in_scale, float out_scale, double* diag) {
+                           float in_scale, float out_scale, int per_stage, double* diag) {
   if (!c || !p || !u || (!dts && nsteps > 0) || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
   if (!c->have_params) return fail(SDNS_ERR_INVALID, "closure parameters not set");
   if (p->precision != SDNS_SINGLE) return fail(SDNS_ERR_UNSUPPORTED, "the closure runs in single precision");
@@ -1265,7 +1268,7 @@ int sdns_closure_rk4_steps(sdns_closure* c, sdns_plan* p, void* u, const double*
     if (p->N[i] != c->n[i]) return fail(SDNS_ERR_INVALID, "plan and closure grids differ");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = rk4_steps<float>(p, u, nullptr, dts, nsteps, nu, diag, nullptr, c, in_scale, out_scale);
+  rc = rk4_steps<float>(p, u, nullptr, dts, nsteps, nu, diag, nullptr, c, in_scale, out_scale, per_stage);
   return rc ? rc : check_launch();
 }
 

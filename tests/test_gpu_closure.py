@@ -145,9 +145,11 @@ def test_zero_closure_equals_plain_step():
     assert torch.equal(ua, ub)
 
 
-def test_closure_step_vs_oracle():
+@pytest.mark.parametrize("per_stage", [True, False])
+def test_closure_step_vs_oracle(per_stage):
     """One RK4 step with the learned Source: GPU increment vs the oracle's, where
-    the oracle evaluates the same network (bf16 storage) on every stage."""
+    the oracle evaluates the same network (bf16 storage) on every stage, or
+    once from the step's initial state and holds it (per_stage False)."""
     from spectraldns_amd._abi import Plan
     from spectraldns_amd.isotropic import random_solenoidal_field
     c, p = make(N0, seed=8, bias_scale=0.0)
@@ -156,7 +158,7 @@ def test_closure_step_vs_oracle():
     u0 = random_solenoidal_field(N0, seed=3, k0=3.0, energy=0.5)
     nu, dt = 0.01, 0.01
     ud = torch.from_numpy(u0).to(torch.complex64).cuda()
-    c.rk4_steps(plan, ud, [dt], nu)
+    c.rk4_steps(plan, ud, [dt], nu, per_stage=per_stage)
     torch.cuda.synchronize()
     g = Grid(N0, (TWO_PI,) * 3, "double", "2/3-rule", True, 0)
     W, b = split(p)
@@ -190,8 +192,9 @@ def test_closure_step_vs_oracle():
     u1 = u0.copy()
     u2 = u0.copy()
     us = u0.copy()
+    held = source(u0)
     for k in range(4):
-        r = g.compute_rhs(us, nu, source=source(us))
+        r = g.compute_rhs(us, nu, source=source(us) if per_stage else held)
         if k < 3:
             us = u1 + bb[k] * dt * r
         u2 = u2 + a[k] * dt * r

@@ -120,3 +120,34 @@ def test_isotropic_generator_properties():
     Ek, bins = g.spectrum(u)
     assert 3 <= int(np.argmax(Ek)) + 1 <= 5
     assert list(hermitian_weights(8)) == [1, 2, 2, 2, 1]
+
+
+def test_closure_symmetry_images_match_analytic_fields():
+    """symmetry(): u'_i(x') = s_i u_{p(i)}(x), tau'_ij(x') = s_i s_j tau_{p(i)p(j)}(x)
+    with x'_i = s_i x_{p(i)} on the periodic grid, checked against analytic fields."""
+    import itertools
+    import torch
+    from spectraldns_amd.closure import symmetry, _PAIRS
+    N = 8
+    xs = 2 * np.pi * np.arange(N) / N
+
+    def u_of(x):  # x: (3, ...) coordinates
+        return np.stack([np.sin(x[1]) * np.cos(2 * x[2]), 2 * np.cos(x[0]) + np.sin(x[2]),
+                         np.sin(x[0] + 2 * x[1])])
+
+    X = np.stack(np.meshgrid(xs, xs, xs, indexing="ij"))
+    u = u_of(X)
+    tau = np.stack([u[i] * u[j] + (i == j) for i, j in _PAIRS])
+    ut, tt = torch.from_numpy(u), torch.from_numpy(tau)
+    for perm in itertools.permutations(range(3)):
+        for signs in ((1, 1, 1), (-1, 1, 1), (1, -1, -1), (-1, -1, 1)):
+            u2, t2 = symmetry(ut, tt, perm, signs)
+            x = np.empty_like(X)
+            for i in range(3):
+                x[perm[i]] = signs[i] * X[i]  # old coordinates of the new grid point X
+            uo = u_of(x)
+            ue = np.stack([signs[i] * uo[perm[i]] for i in range(3)])
+            te = np.stack([signs[i] * signs[j] * (uo[perm[i]] * uo[perm[j]] + (perm[i] == perm[j]))
+                           for i, j in _PAIRS])
+            assert np.allclose(u2.numpy(), ue, atol=1e-12), (perm, signs)
+            assert np.allclose(t2.numpy(), te, atol=1e-12), (perm, signs)

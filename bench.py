@@ -114,6 +114,63 @@ def ensemble_main(args):
     print(json.dumps(out), flush=True)
 
 
+def forced_main(args):
+    """Forced isotropic turbulence, compensated FP32: one step = one RK4 step
+    (u_hat carried as the pair u_hat + u_lo) followed by the large-scale
+    forcing of demo/Isotropic.py:154-184 at |k| <= 2."""
+    import torch
+    from spectraldns_amd._abi import Plan
+    from spectraldns_amd.isotropic import random_solenoidal_field_gpu
+    N = (args.N,) * 3
+    nu, dt, kf = 0.0002, 0.0005, 2.0
+    plan = Plan(N, (2 * np.pi,) * 3, "single", "2/3-rule")
+    t0 = time.perf_counter()
+    u = random_solenoidal_field_gpu(plan, seed=1234, k0=4.0, energy=0.5)
+    ulo = torch.zeros_like(u)
+    e = torch.zeros(2, dtype=torch.float64, device="cuda")
+    plan.energy(u, nu, e)
+    target = 2.0 * float(e[0].item())
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    diag = torch.zeros(2 * max(args.steps, args.warmup, 1), dtype=torch.float64, device="cuda")
+
+    def run(n):
+        for s in range(n):
+            plan.rk4_steps(u, [dt], nu, diag=diag[2 * s:], u_lo=ulo)
+            plan.force(u, target, kf)
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    plan.timing_read()
+    plan.timing_enable(True)
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    plan.timing_enable(False)
+    passes = plan.timing_read()
+    F = N[0] * N[1] * (N[2] // 2 + 1) * 8
+    b_step = 214 * F  # 2/3 rule with compensated u_hat and u2 (hi, lo)
+    sps = args.steps / el
+    d = diag[: 2 * args.steps].cpu().numpy().reshape(-1, 2)
+    out = {
+        "metric": "forced isotropic %d^3 RK4 steps/s (FP32 compensated, FP64 reductions)" % args.N,
+        "value": round(sps, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000 * el / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random-phase solenoidal field generated on the GPU, E(k)~k^4 exp(-2(k/4)^2))",
+        "config": {"workload": "forced isotropic turbulence %d^3, 2/3 rule, RK4, nu=0.0002, dt=0.0005, forcing "
+                               "|k|<=2, compensated u_hat" % args.N, "N": args.N},
+        "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBps": round(b_step * sps / 1e9, 1),
+                          "frac": round(b_step * sps / 1e9 / HBM_PEAK_GBS, 4)},
+        "passes": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 3), "launches": v[1]} for k, v in passes.items()},
+        "energy_first_last": [float(d[0][0]), float(d[-1][0])], "dissipation_last": float(d[-1][1]),
+        "init_s": round(t_init, 2),
+        "memory_GB": round(torch.cuda.max_memory_allocated() / 1e9, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(N, dealias, dom, variants):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when that summary
@@ -149,8 +206,9 @@ def main():
     ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble"),
-                    help="isotropic: the 256^3 box (default); ensemble: the 96-box uneven ensemble")
+    ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble", "forced"),
+                    help="isotropic: the 256^3 box (default); ensemble: the 96-box uneven ensemble; "
+                         "forced: forced isotropic turbulence with compensated FP32 (use --N 1024)")
     ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of graph replay")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first copy a buffer of this size 3 times (reference kernel for HBM counters)")
@@ -161,6 +219,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         ensemble_main(args)
+        return
+    if args.workload == "forced":
+        forced_main(args)
         return
     N = (args.N,) * 3
     if args.calibrate_copy:

@@ -74,3 +74,56 @@ def random_solenoidal_field(N, L=(2 * np.pi,) * 3, seed=1234, k0=4.0, energy=0.5
     e = 0.5 * float(np.sum(w * (u.real ** 2 + u.imag ** 2)))
     u *= np.sqrt(energy / e)
     return u
+
+
+def random_solenoidal_field_gpu(plan, seed=1234, k0=4.0, energy=0.5):
+    """The same construction on the GPU, for boxes whose host generation would
+    take minutes (1024^3): Rogallo amplitudes and basis, phases from torch's
+    seeded device generator (so the draws differ from the PCG64 host field),
+    Hermitian symmetry by a backward/forward round trip with the plan's
+    transforms, projection by the plan's pressure kernel (nu = 0), Nyquist
+    planes and mean mode zeroed, energy scaled with the FP64 energy kernel.
+    Returns a (3, N0, N1, N2//2+1) tensor of the plan's complex dtype."""
+    import torch
+    N, L = plan.N, plan.L
+    Nh = N[2] // 2 + 1
+    dev = plan.device
+    rd, cd = plan.real_dtype, plan.complex_dtype
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    k1 = (torch.fft.fftfreq(N[0], 1.0 / N[0], device=dev, dtype=rd) * (2 * np.pi / L[0])).view(-1, 1, 1)
+    k2 = (torch.fft.fftfreq(N[1], 1.0 / N[1], device=dev, dtype=rd) * (2 * np.pi / L[1])).view(1, -1, 1)
+    k3 = (torch.arange(Nh, device=dev, dtype=rd) * (2 * np.pi / L[2])).view(1, 1, -1)
+    shape = (N[0], N[1], Nh)
+    u = torch.empty((3,) + shape, dtype=cd, device=dev)
+    K2 = k1 * k1 + k2 * k2 + k3 * k3
+    k = torch.sqrt(K2)
+    kk = torch.where(K2 == 0, torch.ones_like(K2), K2)
+    kn = torch.where(k == 0, torch.ones_like(k), k)
+    ksq = torch.sqrt(k1 * k1 + k2 * k2).expand(shape)
+    ksq = torch.where(ksq == 0, torch.ones_like(ksq), ksq)
+    amp = torch.sqrt(k ** 4 * torch.exp(-2.0 * (k / k0) ** 2) / (4.0 * np.pi * kk))
+    del K2, k
+    two_pi = 2 * np.pi
+    th1 = torch.rand(shape, generator=g, device=dev, dtype=rd) * two_pi
+    th2 = torch.rand(shape, generator=g, device=dev, dtype=rd) * two_pi
+    phi = torch.rand(shape, generator=g, device=dev, dtype=rd) * two_pi
+    alpha = amp * torch.cos(phi) * torch.polar(torch.ones_like(th1), th1)
+    beta = amp * torch.sin(phi) * torch.polar(torch.ones_like(th2), th2)
+    del th1, th2, phi, amp
+    u[0] = (alpha * kn * k2 + beta * k1 * k3) / (kn * ksq)
+    u[1] = (beta * k2 * k3 - alpha * kn * k1) / (kn * ksq)
+    u[2] = beta * ksq / kn
+    del alpha, beta, ksq, kn, kk
+    plan.mask_nyquist(u)
+    phys = torch.empty((3,) + tuple(N), dtype=rd, device=dev)
+    plan.backward(u, phys)
+    plan.forward(phys, u)
+    del phys
+    plan.add_pressure_diffusion(u, u, 0.0)  # u -= K (K . u) / K^2 (nu = 0)
+    u[:, 0, 0, 0] = 0
+    plan.mask_nyquist(u)
+    e = torch.zeros(2, dtype=torch.float64, device=dev)
+    plan.energy(u, 0.0, e)
+    u *= float(np.sqrt(energy / float(e[0].item())))
+    return u

@@ -245,6 +245,35 @@ def test_z_pass_variants_fp64(N, rule):
     assert np.array_equal(out[2], out[4]) and np.array_equal(out[3], out[5])
 
 
+@pytest.mark.parametrize("N", [(1024, 32, 64), (32, 64, 1024)])
+def test_long_lines_fp64(N):
+    """1024-point x lines and 512-point half z lines (the line lengths of a
+    1024^3 box), 2/3 rule: RHS and two RK4 steps against the oracle, and every
+    z-pass variant for the z case."""
+    L = (TWO_PI,) * 3
+    nu = 0.005
+    g = Grid(N, L, "double", "2/3-rule")
+    u = field(N, L, seed=29)
+    ref = g.compute_rhs(u, nu)
+    for var in (range(6) if N[2] == 1024 else (0,)):
+        p = plan_of(N, L, "double", "2/3-rule")
+        p.set_variant(2, var)
+        ud = to_gpu(u, torch.complex128)
+        rhs = torch.empty_like(ud)
+        p.compute_rhs(ud, rhs, nu)
+        assert rel(rhs.cpu().numpy(), ref) < 1e-13, var
+    p = plan_of(N, L, "double", "2/3-rule")
+    ud = to_gpu(u, torch.complex128)
+    p.rk4_steps(ud, [0.002, 0.002], nu)
+    assert rel(ud.cpu().numpy(), g.rk4_step(g.rk4_step(u, 0.002, nu), 0.002, nu)) < 1e-12
+    # single precision through the same lines
+    p = plan_of(N, L, "single", "2/3-rule")
+    uf = to_gpu(u, torch.complex64)
+    rhs = torch.empty_like(uf)
+    p.compute_rhs(uf, rhs, nu)
+    assert rel(rhs.cpu().numpy(), ref) < 1e-5
+
+
 def test_compensated_fp32_accumulation():
     """Many small steps: plain FP32 accumulation of u_hat drifts by about one
     rounding per step, the compensated (hi, lo) pair stays at FP32 resolution

@@ -95,7 +95,7 @@ template <typename Rt, int LMAX>
 __global__ void __launch_bounds__(batch_zthreads<Rt>())
     k_batch_p3(const P3Args<Rt>* __restrict__ args, const int* __restrict__ len, const BatchWork* __restrict__ work) {
   using C = cx<Rt>;
-  __shared__ C smem[batch_z_lds<Rt>(LMAX, ZPAD)];
+  __shared__ C smem[batch_z_lds<Rt>(LMAX)];
   const BatchWork w = work[blockIdx.x];
   if (w.box < 0) return;
   const P3Args<Rt> a = args[w.box];
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(batch_zthreads<Rt>())
   case n:                                                      \
     if constexpr (n <= LMAX) {                                 \
       constexpr int E = batch_zE(n), LPB = batch_LPB<Rt>(n);   \
-      C* tw = smem + 3 * LPB * (n + ZPAD);                     \
+      C* tw = smem + 3 * LPB * z3_slot(n, n / E, (int)sizeof(C)); \
       C* rtw = tw + n;                                         \
       table_to_lds(tw, (const C*)a.tw, n);                     \
       table_to_lds(rtw, (const C*)a.rtw, n);                   \

@@ -45,11 +45,11 @@ template <typename Rt, int M, int E_> struct SCfg {
   static_assert(T > 0 && THREADS <= 1024, "no tile fits");
 };
 
-constexpr int ZPAD = 1;
+
 constexpr int z_lpb(int n, int P, int esz) {
   int L = 256 / P;
   if (L < 1) L = 1;
-  while (L > 1 && L * (n + ZPAD) * esz > 32768) L /= 2;
+  while (L > 1 && L * zslot(n, P) * esz > 32768) L /= 2;
   return L;
 }
 template <typename Rt, int n, int E_> struct ZCfg {
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
   constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
-  __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C lds[LPB * zslot(n, P)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
   const long li = valid ? line : 0;
-  C* my = lds + l * (n + ZPAD);
+  C* my = lds + l * zslot(n, P);
   const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
   const long ifield = (long)a.L * a.nkzp;
   LineStage<Rt, n, P> st;
@@ -378,12 +378,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
 // only the last vorticity line is held in registers, so 4E complex values per
 // thread are live instead of 6E.  Three LDS slots per line serve as staging,
 // FFT exchange and parking space.
-constexpr int z3_lpb(int n, int P, int esz) {
-  int L = 256 / P;
-  if (L < 1) L = 1;
-  while (L > 1 && 3 * L * (n + ZPAD) * esz > 40960) L /= 2;
-  return L;
-}
+constexpr int z3_lpb(int n, int P, int esz) { return z3_lines(n, P, esz, z3_slot(n, P, esz)); }
 template <typename Rt, int n, int E_> struct ZBCfg {
   static constexpr int E = E_;
   static constexpr int P = n / E;
@@ -395,7 +390,7 @@ template <typename Rt, int n, int E, int LPB>
 __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>* lds, const cx<Rt>* tw,
                                          const cx<Rt>* rtw) {
   using C = cx<Rt>;
-  constexpr int P = n / E, SL = n + ZPAD;
+  constexpr int P = n / E, SL = z3_slot(n, P, (int)sizeof(cx<Rt>));
   constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
@@ -467,7 +462,7 @@ template <typename Rt, int n, int E>
 __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3b(P3Args<Rt> a) {
   using C = cx<Rt>;
   constexpr int LPB = ZBCfg<Rt, n, E>::LPB;
-  __shared__ C lds[3 * LPB * (n + ZPAD)];
+  __shared__ C lds[3 * LPB * z3_slot(n, n / E, (int)sizeof(cx<Rt>))];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
@@ -483,7 +478,7 @@ __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + s
 // in LDS over the consumed velocity rows, the cross product is formed in
 // registers and the three R2C transforms go back to HBM.  Registers hold 4E
 // complex values and no prefetch.  LDS (dynamic, bytes from zc_lds_bytes):
-//   [6][LPB][nkzp] staging | [LPB][n+ZPAD] exchange | w0/w1 slots if they do
+//   [6][LPB][nkzp] staging | [LPB][zslot(n, P)] exchange | w0/w1 slots if they do
 //   not fit over the staged u rows | tw[n] | rtw[n].
 
 // Copy `bytes` (a multiple of 16) from global src to LDS dst: 16 bytes per
@@ -523,11 +518,11 @@ __device__ __forceinline__ void c2r_pre_row(cx<Rt> (&z)[E], const cx<Rt>* my, in
 struct ZcLayout {
   int stage, xch, om, tw, rtw, total;  // offsets / size in complex elements
 };
-__host__ __device__ inline ZcLayout zc_layout(int n, int lpb, int nkzp) {
+__host__ __device__ inline ZcLayout zc_layout(int n, int P, int lpb, int nkzp) {
   ZcLayout z;
   z.stage = 0;
   z.xch = 6 * lpb * nkzp;
-  const int slot = n + ZPAD;
+  const int slot = zslot(n, P);
   const bool fits = 2 * slot <= 3 * nkzp;  // w0, w1 over the staged u rows of the same block
   z.om = fits ? 0 : z.xch + lpb * slot;
   z.tw = z.xch + lpb * slot + (fits ? 0 : 2 * lpb * slot);
@@ -541,14 +536,14 @@ __host__ __device__ inline ZcLayout zc_layout(int n, int lpb, int nkzp) {
 template <typename Rt, int n, int E, int LPB>
 __global__ void __launch_bounds__(LPB * (n / E), 2) k_p3c(P3Args<Rt> a) {
   using C = cx<Rt>;
-  constexpr int P = n / E, SL = n + ZPAD;
+  constexpr int P = n / E, SL = zslot(n, P);
   static_assert((LPB * P) % 64 == 0, "whole waves");
   constexpr bool WL = 64 % P == 0;
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
   extern __shared__ __align__(16) unsigned char zc_smem[];
   C* sm = (C*)zc_smem;
-  const ZcLayout Z = zc_layout(n, LPB, a.nkzp);
+  const ZcLayout Z = zc_layout(n, P, LPB, a.nkzp);
   C* st = sm + Z.stage;
   C* tw = sm + Z.tw;
   C* rtw = sm + Z.rtw;
@@ -616,7 +611,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
   constexpr bool WL = 64 % P == 0;
   using Fi = LineFFT<Rt, n, E, +1, WL>;
-  __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C lds[LPB * zslot(n, P)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
@@ -625,7 +620,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
   const long li = valid ? line : 0;
-  C* my = lds + l * (n + ZPAD);
+  C* my = lds + l * zslot(n, P);
   const C* in = (const C*)a.in + blockIdx.y * a.in_field + li * a.in_line;
   LineStage<Rt, n, P> st;
   st.load(in, a.nk, valid, t);
@@ -649,7 +644,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   constexpr int P = Cfg::P, LPB = Cfg::LPB;
   constexpr bool WL = 64 % P == 0;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
-  __shared__ C lds[LPB * (n + ZPAD)];
+  __shared__ C lds[LPB * zslot(n, P)];
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
@@ -658,7 +653,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   const int line = blockIdx.x * LPB + l;
   const bool valid = line < a.L;
   const long li = valid ? line : 0;
-  C* my = lds + l * (n + ZPAD);
+  C* my = lds + l * zslot(n, P);
   const C* in = (const C*)a.in + blockIdx.y * a.in_field + li * a.in_line;
   C z[E];
 #pragma unroll

@@ -101,10 +101,10 @@ template <typename Rt> constexpr int batch_strided_lds(int lmax) {
 #undef SDNS_X
   return m;
 }
-// LDS of the z batch kernel: three padded line slots per line + two tables
-template <typename Rt> constexpr int batch_z_lds(int lmax, int zpad) {
+// LDS of the z batch kernel: three line slots (stride zslot) per line + two tables
+template <typename Rt> constexpr int batch_z_lds(int lmax) {
   int m = 0;
-#define SDNS_X(n) if (n <= lmax) m = cmax(m, 3 * batch_LPB<Rt>(n) * (n + zpad) + 2 * n);
+#define SDNS_X(n) if (n <= lmax) m = cmax(m, 3 * batch_LPB<Rt>(n) * z3_slot(n, n / batch_zE(n), (int)sizeof(Rt) * 2) + 2 * n);
   SDNS_BATCH_HALF(SDNS_X)
 #undef SDNS_X
   return m;

@@ -32,7 +32,7 @@ template <typename Rt, int n, int E, int LPB> bool p3c_try(const P3Args<Rt>& a, 
   if constexpr (n < 32 || TH % 64 != 0 || TH > 512) {
     return false;
   } else {
-    const size_t bytes = (size_t)zc_layout(n, LPB, a.nkzp).total * sizeof(cx<Rt>);
+    const size_t bytes = (size_t)zc_layout(n, n / E, LPB, a.nkzp).total * sizeof(cx<Rt>);
     if (bytes > 65536) return false;
     hipLaunchKernelGGL((k_p3c<Rt, n, E, LPB>), dim3((a.L + LPB - 1) / LPB), dim3(TH), bytes, s, a);
     return true;

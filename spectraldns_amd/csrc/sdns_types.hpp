@@ -118,4 +118,33 @@ struct BatchWork {
   int box, tile;
 };
 
+// LDS stride (complex elements) of one z line owned by P threads; the staged
+// spectral line needs n + 1 entries.  With P >= 16 a line fills at least a
+// quarter wave, and a stride of 16 modulo 32 puts the two lines of a fp32
+// half-wave 32 banks apart, so their contiguous accesses share no bank.
+constexpr int zslot(int n, int P) { return P >= 16 ? n + 1 + ((16 - (n + 1) % 32) + 32) % 32 : n + 1; }
+
+// Lines per block of the LDS-assisted z scheme (three slots of stride sl per
+// line within 40 KB), and the slot stride it uses: the padded stride when it
+// keeps the lines per block and either the blocks that fit in a CU's 160 KB of
+// LDS (twiddle tables included) or at least five of them.  Measured on the
+// 256^3 z passes: padding gains 16% at n = 128 (6 -> 5 blocks per CU) and
+// loses 15% at n = 192 (4 -> 3 blocks per CU).
+constexpr int z3_lines(int n, int P, int esz, int sl) {
+  int L = 256 / P;
+  if (L < 1) L = 1;
+  while (L > 1 && 3 * L * sl * esz > 40960) L /= 2;
+  return L;
+}
+constexpr int z3_blocks_per_cu(int n, int P, int esz, int sl) {
+  return 163840 / ((3 * z3_lines(n, P, esz, sl) * sl + 2 * n) * esz);
+}
+constexpr int z3_slot(int n, int P, int esz) {
+  return z3_lines(n, P, esz, zslot(n, P)) == z3_lines(n, P, esz, n + 1) &&
+                 (z3_blocks_per_cu(n, P, esz, zslot(n, P)) >= z3_blocks_per_cu(n, P, esz, n + 1) ||
+                  z3_blocks_per_cu(n, P, esz, zslot(n, P)) >= 5)
+             ? zslot(n, P)
+             : n + 1;
+}
+
 }  // namespace sdns

@@ -233,6 +233,37 @@ template <bool WAVE> __device__ __forceinline__ void lds_barrier() {
   }
 }
 
+// ------------------------------------------------------------------ buffer access
+// Raw buffer loads and stores (stride 0, byte offsets).  An access whose
+// offset is at or beyond the descriptor's byte count reads zeros / writes
+// nothing, so a masked element costs a select of its offset instead of a
+// branch.  A descriptor covers one block's span of lines (at most 2^31 - 1
+// bytes); BUF_OOB is an offset beyond every such span.
+constexpr int BUF_OOB = 0x7ff00000;
+typedef unsigned int buf_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int buf_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, long bytes) {
+  const int nb = bytes > 0x7fffffffL ? 0x7fffffff : (int)bytes;
+  // word 3 = 0x00020000: 32-bit data format, the gfx9 raw-buffer setting
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nb, 0x00020000);
+}
+template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(C) == 8) {
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  } else {
+    static_assert(sizeof(C) == 16, "complex of float or double");
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+}
+template <typename C> __device__ __forceinline__ void buf_store(C x, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(C) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u32x2, x), r, off, 0, 0);
+  } else {
+    static_assert(sizeof(C) == 16, "complex of float or double");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, off, 0, 0);
+  }
+}
+
 // tw[m] = exp(-2 pi i m / M), m in [0, M): the forward twiddle table of the
 // line length; the inverse direction uses its conjugate.  WAVE: the P threads
 // of a line are in one wave (exchanges need no block barrier).

@@ -272,6 +272,16 @@ __device__ __forceinline__ void c2r_pre_lds(cx<Rt> (&z)[E], const cx<Rt>* my, co
   }
 }
 
+// Bin k of the real transform from the half-length FFT values A = Z[k] and
+// Bz = Z[(n - k) mod n], w = rtw[k].
+template <typename C> __device__ __forceinline__ C r2c_bin(C A, C Bz, C w) {
+  using Rt = decltype(A.x);
+  const C s = mk<C>(A.x + Bz.x, A.y - Bz.y);   // A + conj(B)
+  const C d = mk<C>(A.x - Bz.x, A.y + Bz.y);   // A - conj(B)
+  const C wd = cmul(w, d);
+  return mk<C>(Rt(0.5) * (s.x + wd.y), Rt(0.5) * (s.y - wd.x));
+}
+
 // Post-process of the forward half-length FFT held in z (natural positions
 // t + P e); `my` is this line's LDS area.  Writes kept bins k < nk (and the
 // Nyquist bin n when nk > n).
@@ -286,14 +296,30 @@ __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __r
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + P * e;
-    const C A = z[e];
-    const C Bz = my[(n - k) % n];
-    const C s = mk<C>(A.x + Bz.x, A.y - Bz.y);   // A + conj(B)
-    const C d = mk<C>(A.x - Bz.x, A.y + Bz.y);   // A - conj(B)
-    const C wd = cmul(rtw[k], d);
-    const C X = mk<C>(Rt(0.5) * (s.x + wd.y), Rt(0.5) * (s.y - wd.x));
-    if (valid && k < nk) out[k] = cscale(X, scale);
-    if (valid && k == 0 && nk > n) out[n] = mk<C>((A.x - A.y) * scale, Rt(0));
+    const C X = r2c_bin(z[e], my[(n - k) % n], rtw[k]);
+    if (valid && t < nk - P * e) out[k] = cscale(X, scale);
+    if (valid && k == 0 && nk > n) out[n] = mk<C>((z[e].x - z[e].y) * scale, Rt(0));
+  }
+  lds_barrier<WAVE>();
+}
+
+// r2c_post writing through a buffer descriptor: `base` is the byte offset of
+// this line in the descriptor's span (lines outside the span are dropped by
+// the range check); unit scale.
+template <typename Rt, int n, int E, bool WAVE = false>
+__device__ __forceinline__ void r2c_post_buf(cx<Rt> (&z)[E], cx<Rt>* my, __amdgpu_buffer_rsrc_t rs, int base, int nk,
+                                             const cx<Rt>* __restrict__ rtw, int t) {
+  using C = cx<Rt>;
+  constexpr int P = n / E, ES = (int)sizeof(C);
+#pragma unroll
+  for (int e = 0; e < E; ++e) my[t + P * e] = z[e];
+  lds_barrier<WAVE>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + P * e;
+    const C X = r2c_bin(z[e], my[(n - k) % n], rtw[k]);
+    buf_store<C>(X, rs, t < nk - P * e ? base + ES * k : BUF_OOB);
+    if (k == 0 && nk > n) buf_store<C>(mk<C>(z[e].x - z[e].y, Rt(0)), rs, base + ES * n);
   }
   lds_barrier<WAVE>();
 }
@@ -308,9 +334,17 @@ struct LineStage {
   __device__ __forceinline__ void load(const cx<Rt>* __restrict__ in, int nk, bool valid, int t) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int i = t + P * k;
-      r[k] = (valid && i < nk) ? in[i] : czero<cx<Rt>>();
+      // (t < nk - P k rather than t + P k < nk: the bound is uniform, so the
+      // compare needs no per-element index register)
+      r[k] = (valid && t < nk - P * k) ? in[t + P * k] : czero<cx<Rt>>();
     }
+  }
+  // Through a buffer descriptor: `base` = byte offset of this line in the
+  // descriptor's span; bins >= nk and lines outside the span read zero.
+  __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rs, int base, int nk, int t) {
+    constexpr int ES = (int)sizeof(cx<Rt>);
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = buf_load<cx<Rt>>(rs, t < nk - P * k ? base + ES * (t + P * k) : BUF_OOB);
   }
   __device__ __forceinline__ void store(cx<Rt>* my, int t) const {
 #pragma unroll
@@ -336,14 +370,15 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
   table_to_lds(rtw, (const C*)a.rtw, n);
   __syncthreads();
   const int t = threadIdx.x % P, l = threadIdx.x / P;
-  const int line = blockIdx.x * LPB + l;
-  const bool valid = line < a.L;
-  const long li = valid ? line : 0;
+  const int line0 = blockIdx.x * LPB;
   C* my = lds + l * zslot(n, P);
-  const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
+  const int nl = min(LPB, a.L - line0);
   const long ifield = (long)a.L * a.nkzp;
+  const C* inb = (const C*)a.w2 + (long)line0 * a.nkzp;
+  const long ibytes = (long)nl * a.nkzp * (long)sizeof(C);
+  const int ibase = l * a.nkzp * (int)sizeof(C);
   LineStage<Rt, n, P> st;
-  st.load(in, a.nkz, valid, t);
+  st.load_buf(buf_rsrc(inb, ibytes), ibase, a.nkz, t);
   C ph[6][E];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
@@ -351,7 +386,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
     lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(ph[f], my, rtw, t);
     lds_barrier<WL>();
-    if (f < 5) st.load(in + (f + 1) * ifield, a.nkz, valid, t);  // next field in flight during the FFT
+    if (f < 5) st.load_buf(buf_rsrc(inb + (f + 1) * ifield, ibytes), ibase, a.nkz, t);  // next field in flight during the FFT
     Fi::run(ph[f], t, my, 1, tw);
   }
   // u x w on the packed points (real part: x[2m], imaginary part: x[2m+1]);
@@ -364,12 +399,14 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + st
     ph[1][e] = mk<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
     ph[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
   }
-  C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
   const long ofield = (long)a.L * a.nkzop;
+  C* outb = (C*)a.w3 + (long)line0 * a.nkzop;
+  const long obytes = (long)nl * a.nkzop * (long)sizeof(C);
+  const int obase = l * a.nkzop * (int)sizeof(C);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     Ff::run(ph[i], t, my, 1, tw);
-    r2c_post<Rt, n, E, WL>(ph[i], my, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+    r2c_post_buf<Rt, n, E, WL>(ph[i], my, buf_rsrc(outb + i * ofield, obytes), obase, a.nkzo, rtw, t);
   }
 }
 
@@ -396,16 +433,18 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   using Ff = LineFFT<Rt, n, E, -1, WL>;
   __syncthreads();  // twiddle tables written by the caller
   const int t = threadIdx.x % P, l = threadIdx.x / P;
-  const int line = line0 + l;
-  const bool valid = line < a.L;
-  const long li = valid ? line : 0;
   C* s0 = lds + (3 * l) * SL;
   C* s1 = s0 + SL;
   C* s2 = s1 + SL;
-  const C* __restrict__ in = (const C*)a.w2 + li * a.nkzp;
+  // buffer descriptors over this block's lines (lines past a.L read zero and
+  // are not written)
+  const int nl = min(LPB, a.L - line0);
   const long ifield = (long)a.L * a.nkzp;
+  const C* inb = (const C*)a.w2 + (long)line0 * a.nkzp;
+  const long ibytes = (long)nl * a.nkzp * (long)sizeof(C);
+  const int ibase = l * a.nkzp * (int)sizeof(C);
   LineStage<Rt, n, P> st;
-  st.load(in, a.nkz, valid, t);
+  st.load_buf(buf_rsrc(inb, ibytes), ibase, a.nkz, t);
   C U[3][E], W[E];
   // velocity lines -> registers (staging and exchange in slot 0)
 #pragma unroll
@@ -414,7 +453,7 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
     lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(U[f], s0, rtw, t);
     lds_barrier<WL>();
-    st.load(in + (f + 1) * ifield, a.nkz, valid, t);
+    st.load_buf(buf_rsrc(inb + (f + 1) * ifield, ibytes), ibase, a.nkz, t);
     Fi::run(U[f], t, s0, 1, tw);
   }
   // w0 -> slot 1, w1 -> slot 2 (each slot is its own staging and exchange area)
@@ -425,7 +464,7 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
     lds_barrier<WL>();
     c2r_pre_lds<Rt, n, E>(W, sl, rtw, t);
     lds_barrier<WL>();
-    st.load(in + (4 + f) * ifield, a.nkz, valid, t);
+    st.load_buf(buf_rsrc(inb + (4 + f) * ifield, ibytes), ibase, a.nkz, t);
     Fi::run(W, t, sl, 1, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) sl[t + P * e] = W[e];
@@ -448,13 +487,15 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
     U[2][e] = mk<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
   }
   lds_barrier<WL>();
-  C* __restrict__ out = (C*)a.w3 + li * a.nkzop;
   const long ofield = (long)a.L * a.nkzop;
+  C* outb = (C*)a.w3 + (long)line0 * a.nkzop;
+  const long obytes = (long)nl * a.nkzop * (long)sizeof(C);
+  const int obase = l * a.nkzop * (int)sizeof(C);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     C* sl = i == 0 ? s0 : (i == 1 ? s1 : s2);
     Ff::run(U[i], t, sl, 1, tw);
-    r2c_post<Rt, n, E, WL>(U[i], sl, out + i * ofield, a.nkzo, valid, rtw, t, Rt(1));
+    r2c_post_buf<Rt, n, E, WL>(U[i], sl, buf_rsrc(outb + i * ofield, obytes), obase, a.nkzo, rtw, t);
   }
 }
 

@@ -807,136 +807,173 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
   const Rt K1 = a.ky[iy], K2c = a.kz[kz];
   const bool nyq_col = a.mask_nyquist && (iy == a.N1 / 2 || kz == a.N2 / 2);
   double esum = 0.0, osum = 0.0;
+  // Elements go in chunks of G: all loads of a chunk are issued before any of
+  // its stores, so G elements' state is in flight at once instead of one
+  // round trip per array and element.  An element's rows are written only by
+  // that element, so no load of a chunk can see a store of the same chunk.
+  constexpr int G = (E >= 24 || TH >= 1024 || MODE == P5_RHS) ? 1 : 2;
+  static_assert(E % G == 0, "chunk must divide the points per thread");
+  const C* ust = (const C*)(MODE == P5_RK0 || MODE == P5_RHS ? a.A : a.B);
+  const C* src = (const C*)a.source;
+  const C* Ain = (const C*)a.A;
+  const C* Cin = (const C*)a.Cacc;
+  const C* Lin = (const C*)(MODE == P5_RK0 ? a.Alo : a.Clo);
+  constexpr bool LOAD_A = MODE == P5_RK12;
+  constexpr bool LOAD_C = MODE == P5_RK12 || MODE == P5_RK3;
+  constexpr bool LOAD_L = COMP && MODE != P5_RHS;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int p = t + P * e;
-    int ix;
-    bool kept;
-    if (M == a.N0) {
-      ix = p;
-      const int k = p < a.N0 / 2 ? p : p - a.N0;
-      kept = k >= -a.x.neg && k <= a.x.pos;
-    } else {
-      ix = bin_row(p, M, a.x);
-      kept = true;
-    }
-    if (!valid || ix < 0) continue;
-    const long off = (long)ix * plane + q;
-    if constexpr (MODE == P5_RHS) {
-      if (a.conv_only) {
-        const bool cv = colconv && kept;
-        C* rhs = (C*)a.rhs;
-        rhs[off] = cv ? cscale(X0[e], a.scale) : czero<C>();
-        rhs[fs + off] = cv ? cscale(X1[e], a.scale) : czero<C>();
-        rhs[2 * fs + off] = cv ? cscale(X2[e], a.scale) : czero<C>();
-        continue;
+  for (int e0 = 0; e0 < E; e0 += G) {
+    int ixs[G];
+    bool keeps[G];
+    C su[G][3], sr[G][3], xa[G][3], xc[G][3], xl[G][3];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int p = t + P * (e0 + g);
+      int ix;
+      bool kept;
+      if (M == a.N0) {
+        ix = p;
+        const int k = p < a.N0 / 2 ? p : p - a.N0;
+        kept = k >= -a.x.neg && k <= a.x.pos;
+      } else {
+        ix = bin_row(p, M, a.x);
+        kept = true;
+      }
+      ixs[g] = valid ? ix : -1;
+      keeps[g] = kept;
+      if (ixs[g] < 0) continue;
+      if constexpr (MODE == P5_RHS) {
+        if (a.conv_only) continue;
+      }
+      const long off = (long)ix * plane + q;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        su[g][i] = ust[i * fs + off];
+        if (src) sr[g][i] = src[i * fs + off];
+        if constexpr (LOAD_A) xa[g][i] = Ain[i * fs + off];
+        if constexpr (LOAD_C) xc[g][i] = Cin[i * fs + off];
+        if constexpr (LOAD_L) xl[g][i] = Lin[i * fs + off];
       }
     }
-    const bool conv = colconv && kept && !(nyq_col || (a.mask_nyquist && ix == a.N0 / 2));
-    C r0 = conv ? cscale(X0[e], a.scale) : czero<C>();
-    C r1 = conv ? cscale(X1[e], a.scale) : czero<C>();
-    C r2 = conv ? cscale(X2[e], a.scale) : czero<C>();
-    const Rt K0 = a.kx[ix];
-    Rt k2 = K0 * K0;
-    k2 += K1 * K1;
-    k2 += K2c * K2c;
-    const Rt den = k2 == Rt(0) ? Rt(1) : k2;
-    const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2c / den;
-    // P_hat = sum_i rhs_i K_i/K^2 ; rhs_i -= P_hat K_i ; rhs -= (nu K^2) u_hat  (NS.py:208-215)
-    C ph = mk<C>(r0.x * o0 + r1.x * o1, r0.y * o0 + r1.y * o1);
-    ph = mk<C>(ph.x + r2.x * o2, ph.y + r2.y * o2);
-    r0 = mk<C>(r0.x - ph.x * K0, r0.y - ph.y * K0);
-    r1 = mk<C>(r1.x - ph.x * K1, r1.y - ph.y * K1);
-    r2 = mk<C>(r2.x - ph.x * K2c, r2.y - ph.y * K2c);
-    const Rt z = a.nu * k2;
-    const C* ust = (const C*)(MODE == P5_RK0 || MODE == P5_RHS ? a.A : a.B);
-    const C s0 = ust[off], s1 = ust[fs + off], s2 = ust[2 * fs + off];
-    r0 = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
-    r1 = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
-    r2 = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
-    if (a.source) {  // rhs += Source (NS.py:256-258), every mode
-      const C* sr = (const C*)a.source;
-      r0 = cadd(r0, sr[off]); r1 = cadd(r1, sr[fs + off]); r2 = cadd(r2, sr[2 * fs + off]);
-    }
-    if constexpr (MODE == P5_RHS) {
-      C* rhs = (C*)a.rhs;
-      rhs[off] = r0; rhs[fs + off] = r1; rhs[2 * fs + off] = r2;
-      if (a.phat) ((C*)a.phat)[off] = ph;
-    } else {
-      C* A = (C*)a.A;
-      C* B = (C*)a.B;
-      C* Cc = (C*)a.Cacc;
-      const Rt bdt = a.bdt, adt = a.adt;
-      auto upd = [](C base, C r, Rt s) { return mk<C>(base.x + s * r.x, base.y + s * r.y); };
-      if constexpr (!COMP) {
-        if constexpr (MODE == P5_RK0) {
-          // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
-          B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
-          Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
-        } else if constexpr (MODE == P5_RK12) {
-          const C b0 = A[off], b1 = A[fs + off], b2 = A[2 * fs + off];
-          B[off] = upd(b0, r0, bdt); B[fs + off] = upd(b1, r1, bdt); B[2 * fs + off] = upd(b2, r2, bdt);
-          const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
-          Cc[off] = upd(c0, r0, adt); Cc[fs + off] = upd(c1, r1, adt); Cc[2 * fs + off] = upd(c2, r2, adt);
-        } else {
-          const C c0 = Cc[off], c1 = Cc[fs + off], c2 = Cc[2 * fs + off];
-          const C n0 = upd(c0, r0, adt), n1 = upd(c1, r1, adt), n2 = upd(c2, r2, adt);
-          A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
-          if (a.phat) ((C*)a.phat)[off] = ph;
-          if (a.partials) {
-            const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
-            const double v[6] = {n0.x, n0.y, n1.x, n1.y, n2.x, n2.y};
-            energy_terms(v, K0, K1, K2c, w, esum, osum);
-          }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int e = e0 + g;
+      const int ix = ixs[g];
+      const bool kept = keeps[g];
+      if (ix < 0) continue;
+      const long off = (long)ix * plane + q;
+      if constexpr (MODE == P5_RHS) {
+        if (a.conv_only) {
+          const bool cv = colconv && kept;
+          C* rhs = (C*)a.rhs;
+          rhs[off] = cv ? cscale(X0[e], a.scale) : czero<C>();
+          rhs[fs + off] = cv ? cscale(X1[e], a.scale) : czero<C>();
+          rhs[2 * fs + off] = cv ? cscale(X2[e], a.scale) : czero<C>();
+          continue;
         }
+      }
+      const bool conv = colconv && kept && !(nyq_col || (a.mask_nyquist && ix == a.N0 / 2));
+      C r0 = conv ? cscale(X0[e], a.scale) : czero<C>();
+      C r1 = conv ? cscale(X1[e], a.scale) : czero<C>();
+      C r2 = conv ? cscale(X2[e], a.scale) : czero<C>();
+      const Rt K0 = a.kx[ix];
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2c * K2c;
+      const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+      const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2c / den;
+      // P_hat = sum_i rhs_i K_i/K^2 ; rhs_i -= P_hat K_i ; rhs -= (nu K^2) u_hat  (NS.py:208-215)
+      C ph = mk<C>(r0.x * o0 + r1.x * o1, r0.y * o0 + r1.y * o1);
+      ph = mk<C>(ph.x + r2.x * o2, ph.y + r2.y * o2);
+      r0 = mk<C>(r0.x - ph.x * K0, r0.y - ph.y * K0);
+      r1 = mk<C>(r1.x - ph.x * K1, r1.y - ph.y * K1);
+      r2 = mk<C>(r2.x - ph.x * K2c, r2.y - ph.y * K2c);
+      const Rt z = a.nu * k2;
+      const C s0 = su[g][0], s1 = su[g][1], s2 = su[g][2];
+      r0 = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
+      r1 = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
+      r2 = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
+      if (src) {  // rhs += Source (NS.py:256-258), every mode
+        r0 = cadd(r0, sr[g][0]); r1 = cadd(r1, sr[g][1]); r2 = cadd(r2, sr[g][2]);
+      }
+      if constexpr (MODE == P5_RHS) {
+        C* rhs = (C*)a.rhs;
+        rhs[off] = r0; rhs[fs + off] = r1; rhs[2 * fs + off] = r2;
+        if (a.phat) ((C*)a.phat)[off] = ph;
       } else {
-        C* Alo = (C*)a.Alo;
-        C* Clo = (C*)a.Clo;
-        const C rr[3] = {r0, r1, r2};
-        if constexpr (MODE == P5_RK0) {
-          const C ss[3] = {s0, s1, s2};
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const long o = i * fs + off;
-            B[o] = upd(ss[i], rr[i], bdt);
-            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
-            Rt ex, ey;
-            const Rt hx = two_sum(ss[i].x, yx, ex), hy = two_sum(ss[i].y, yy, ey);
-            const C l = Alo[o];
-            Cc[o] = mk<C>(hx, hy);
-            Clo[o] = mk<C>(l.x + ex, l.y + ey);
-          }
-        } else if constexpr (MODE == P5_RK12) {
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const long o = i * fs + off;
-            B[o] = upd(A[o], rr[i], bdt);
-            const C c = Cc[o], l = Clo[o];
-            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
-            Rt ex, ey;
-            const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
-            Cc[o] = mk<C>(hx, hy);
-            Clo[o] = mk<C>(l.x + ex, l.y + ey);
+        C* A = (C*)a.A;
+        C* B = (C*)a.B;
+        C* Cc = (C*)a.Cacc;
+        const Rt bdt = a.bdt, adt = a.adt;
+        auto upd = [](C base, C r, Rt s) { return mk<C>(base.x + s * r.x, base.y + s * r.y); };
+        if constexpr (!COMP) {
+          if constexpr (MODE == P5_RK0) {
+            // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
+            B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
+            Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
+          } else if constexpr (MODE == P5_RK12) {
+            B[off] = upd(xa[g][0], r0, bdt); B[fs + off] = upd(xa[g][1], r1, bdt); B[2 * fs + off] = upd(xa[g][2], r2, bdt);
+            Cc[off] = upd(xc[g][0], r0, adt); Cc[fs + off] = upd(xc[g][1], r1, adt);
+            Cc[2 * fs + off] = upd(xc[g][2], r2, adt);
+          } else {
+            const C n0 = upd(xc[g][0], r0, adt), n1 = upd(xc[g][1], r1, adt), n2 = upd(xc[g][2], r2, adt);
+            A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
+            if (a.phat) ((C*)a.phat)[off] = ph;
+            if (a.partials) {
+              const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+              const double v[6] = {n0.x, n0.y, n1.x, n1.y, n2.x, n2.y};
+              energy_terms(v, K0, K1, K2c, w, esum, osum);
+            }
           }
         } else {
-          double v[6];
+          C* Alo = (C*)a.Alo;
+          C* Clo = (C*)a.Clo;
+          const C rr[3] = {r0, r1, r2};
+          if constexpr (MODE == P5_RK0) {
+            const C ss[3] = {s0, s1, s2};
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const long o = i * fs + off;
-            const C c = Cc[o], l = Clo[o];
-            const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
-            Rt ex, ey;
-            const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
-            const Rt lx = l.x + ex, ly = l.y + ey;
-            Rt rx, ry;  // renormalise: hi = fl(h + l), lo = exact remainder
-            const Rt nx = two_sum(hx, lx, rx), ny = two_sum(hy, ly, ry);
-            A[o] = mk<C>(nx, ny);
-            Alo[o] = mk<C>(rx, ry);
-            v[2 * i] = (double)nx + (double)rx;
-            v[2 * i + 1] = (double)ny + (double)ry;
+            for (int i = 0; i < 3; ++i) {
+              const long o = i * fs + off;
+              B[o] = upd(ss[i], rr[i], bdt);
+              const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+              Rt ex, ey;
+              const Rt hx = two_sum(ss[i].x, yx, ex), hy = two_sum(ss[i].y, yy, ey);
+              const C l = xl[g][i];
+              Cc[o] = mk<C>(hx, hy);
+              Clo[o] = mk<C>(l.x + ex, l.y + ey);
+            }
+          } else if constexpr (MODE == P5_RK12) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const long o = i * fs + off;
+              B[o] = upd(xa[g][i], rr[i], bdt);
+              const C c = xc[g][i], l = xl[g][i];
+              const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+              Rt ex, ey;
+              const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
+              Cc[o] = mk<C>(hx, hy);
+              Clo[o] = mk<C>(l.x + ex, l.y + ey);
+            }
+          } else {
+            double v[6];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const long o = i * fs + off;
+              const C c = xc[g][i], l = xl[g][i];
+              const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
+              Rt ex, ey;
+              const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
+              const Rt lx = l.x + ex, ly = l.y + ey;
+              Rt rx, ry;  // renormalise: hi = fl(h + l), lo = exact remainder
+              const Rt nx = two_sum(hx, lx, rx), ny = two_sum(hy, ly, ry);
+              A[o] = mk<C>(nx, ny);
+              Alo[o] = mk<C>(rx, ry);
+              v[2 * i] = (double)nx + (double)rx;
+              v[2 * i + 1] = (double)ny + (double)ry;
+            }
+            if (a.phat) ((C*)a.phat)[off] = ph;
+            if (a.partials) energy_terms(v, K0, K1, K2c, (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0, esum, osum);
           }
-          if (a.phat) ((C*)a.phat)[off] = ph;
-          if (a.partials) energy_terms(v, K0, K1, K2c, (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0, esum, osum);
         }
       }
     }
@@ -962,8 +999,23 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
   }
 }
 
+// Waves per SIMD that whole blocks of `threads` fill at a register-bound
+// occupancy of `occ` (a CU holds 4 occ waves): an 8-wave block at occ 3 gets
+// one block per CU, as at occ 2, so the larger register budget costs nothing.
+constexpr int occ_blocks(int occ, int threads) {
+  const int bw = threads / 64;
+  const int w = (4 * occ) / bw * bw / 4;
+  return w < 1 ? 1 : w;
+}
+// FP32 kernels are held to at least 2 waves per SIMD (256 registers): past
+// that the compiler would move into the AGPR half of the register file and
+// halve the occupancy.
+template <typename Rt, int M, int E> constexpr int p5_occ() {
+  const int o = occ_blocks(occ_for(3 * E, esz_of<Rt>()), SCfg<Rt, M, E>::THREADS);
+  return sizeof(Rt) == 4 && o < 2 ? 2 : o;
+}
 template <typename Rt, int M, int E, int MODE, bool COMP = false>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p5(P5Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p5_occ<Rt, M, E>())) k_p5(P5Args<Rt> a) {
   using C = cx<Rt>;
   constexpr int T = SCfg<Rt, M, E>::T;
   __shared__ C lds[M * T];

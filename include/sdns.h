@@ -135,6 +135,9 @@ int sdns_spectrum(sdns_plan* plan, const void* u_hat, int nbins, double* out);
  * RK stage, 5: other launches) and clears the record. */
 int sdns_timing_enable(sdns_plan* plan, int enable);
 int sdns_timing_read(sdns_plan* plan, double ms[6], int count[6]);
+/* Pass ids recorded while timing is enabled (bit i = pass id i; default 63,
+ * every pass).  Fewer events keep the timed launches back to back. */
+int sdns_timing_passes(sdns_plan* plan, int mask);
 
 /* ------------------------------------------------------------------------
  * Batched ensembles of independent boxes.  No counterpart in the reference,

@@ -42,6 +42,7 @@ _SIGS = {
     "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "sdns_timing_passes": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, _P]),
@@ -322,6 +323,11 @@ class Plan:
 
     def timing_enable(self, on=True):
         _check(self._lib.sdns_timing_enable(self._h, int(bool(on))))
+
+    def timing_passes(self, names=None):
+        """Record only these passes (names from PASS_NAMES; None: all)."""
+        mask = 63 if names is None else sum(1 << self.PASS_NAMES.index(n) for n in names)
+        _check(self._lib.sdns_timing_passes(self._h, mask))
 
     def timing_read(self):
         """{pass name: (milliseconds, launches)} since the last read."""

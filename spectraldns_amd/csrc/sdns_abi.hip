@@ -102,6 +102,8 @@ struct sdns_plan {
   int max_blocks = 0;
   // per-pass event timing
   bool timing = false;
+  int timing_mask = 63;  // pass ids recorded while timing (bit i = pass i)
+  bool cur_rec = false;
   struct Rec { int id; hipEvent_t a, b; };
   std::vector<Rec> rec;
   std::vector<hipEvent_t> pool;
@@ -248,13 +250,15 @@ hipEvent_t pool_event(sdns_plan* p) {
   return e;
 }
 void tstart(sdns_plan* p, int id) {
-  if (!p->timing) return;
+  p->cur_rec = p->timing && ((p->timing_mask >> id) & 1);
+  if (!p->cur_rec) return;
   p->cur_a = pool_event(p);
   p->cur_id = id;
   (void)hipEventRecord(p->cur_a, p->stream);
 }
 void tstop(sdns_plan* p) {
-  if (!p->timing) return;
+  if (!p->cur_rec) return;
+  p->cur_rec = false;
   hipEvent_t b = pool_event(p);
   (void)hipEventRecord(b, p->stream);
   p->rec.push_back({p->cur_id, p->cur_a, b});
@@ -985,6 +989,13 @@ int sdns_spectrum(sdns_plan* p, const void* u, int nbins, double* out) {
 int sdns_timing_enable(sdns_plan* p, int enable) {
   if (!p) return fail(SDNS_ERR_INVALID, "null plan");
   p->timing = enable != 0;
+  return 0;
+}
+
+int sdns_timing_passes(sdns_plan* p, int mask) {
+  if (!p) return fail(SDNS_ERR_INVALID, "null plan");
+  if (mask < 0 || mask > 63) return fail(SDNS_ERR_INVALID, "pass mask %d", mask);
+  p->timing_mask = mask;
   return 0;
 }
 

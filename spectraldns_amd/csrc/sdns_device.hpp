@@ -123,8 +123,17 @@ template <int R1, int R2, int DIR, typename C> __device__ __forceinline__ void d
     sfor<R1>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
       if constexpr (n2 * k1 != 0) {
-        constexpr cs_pair w = cexp2pi((long)DIR * n2 * k1, R);
-        a[k1] = cmul(a[k1], mk<C>((Rt)w.c, (Rt)w.s));
+        constexpr long m = (((long)DIR * n2 * k1) % R + R) % R;
+        if constexpr ((4 * m) % R == 0) {
+          // quarter turns i^q: a swap and a sign instead of a complex product
+          constexpr int q = (int)(4 * m / R);
+          if constexpr (q == 1) a[k1] = cmuli<1>(a[k1]);
+          else if constexpr (q == 2) a[k1] = mk<C>(-a[k1].x, -a[k1].y);
+          else if constexpr (q == 3) a[k1] = cmuli<-1>(a[k1]);
+        } else {
+          constexpr cs_pair w = cexp2pi(m, R);
+          a[k1] = cmul(a[k1], mk<C>((Rt)w.c, (Rt)w.s));
+        }
       }
       y[n2][k1] = a[k1];
     });

@@ -149,6 +149,39 @@ template <int R1, int R2, int DIR, typename C> __device__ __forceinline__ void d
   }
 }
 
+// Prime-factor (Good-Thomas) split R = R1*R2 with gcd(R1, R2) = 1: input
+// n = (n1 R2 + n2 R1) mod R, output k = (k1 R2 t2 + k2 R1 t1) mod R with
+// t2 = R2^-1 mod R1 and t1 = R1^-1 mod R2; then
+// X[k] = sum_n2 W_R2^(n2 k2) sum_n1 x[n] W_R1^(n1 k1), with no twiddles
+// between the two stages.
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+template <int R1, int R2, int DIR, typename C> __device__ __forceinline__ void dft_pfa(C (&v)[R1 * R2]) {
+  constexpr int R = R1 * R2, t2 = inv_mod(R2 % R1, R1), t1 = inv_mod(R1 % R2, R2);
+  C y[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    C a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[(n1 * R2 + n2 * R1) % R];
+    dft<R1, DIR>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[n2][k1] = a[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    C b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2][k1];
+    dft<R2, DIR>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[(k1 * R2 * t2 + k2 * R1 * t1) % R] = b[k2];
+  }
+}
+
 template <int R, int DIR, typename C> __device__ __forceinline__ void dft(C (&v)[R]) {
   if constexpr (R == 1) {
   } else if constexpr (R == 2) {
@@ -158,15 +191,15 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dft(C (&v)
   } else if constexpr (R == 4) {
     dft4<DIR>(v[0], v[1], v[2], v[3]);
   } else if constexpr (R == 6) {
-    dft_ct<2, 3, DIR>(v);
+    dft_pfa<2, 3, DIR>(v);
   } else if constexpr (R == 8) {
     dft_ct<2, 4, DIR>(v);
   } else if constexpr (R == 12) {
-    dft_ct<3, 4, DIR>(v);
+    dft_pfa<3, 4, DIR>(v);
   } else if constexpr (R == 16) {
     dft_ct<4, 4, DIR>(v);
   } else if constexpr (R == 24) {
-    dft_ct<3, 8, DIR>(v);
+    dft_pfa<3, 8, DIR>(v);
   } else {
     static_assert(R == 0, "unsupported radix");
   }

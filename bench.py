@@ -228,7 +228,7 @@ def main():
                     help="first copy a buffer of this size 3 times (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":

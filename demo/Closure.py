@@ -1,4 +1,4 @@
-"""Learned subgrid closure: trained on box-filtered forced DNS, judged by it.
+"""Learned subgrid closure: trained on box-filtered forced DNS, evaluated against it.
 
 All steps run on the GPU through libsdns_hip:
   1. forced isotropic DNS at N^3 (FP32, 2/3 rule, seeded random-phase start,

@@ -113,6 +113,22 @@ int sdns_rk4_steps(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_se
 /* Kinetic energy and dissipation of u_hat in FP64 into the device array out[2]. */
 int sdns_energy(sdns_plan* plan, const void* u_hat, double nu, double* out);
 
+/* Array updates of the ForwardEuler, AB2 and BS5 (adaptiveRK) integrators
+ * (spectralDNS/maths/integrators.py:66-75 stage and solution sums, :161-175
+ * u0 += rhs*dt and AB2), which run with sdns_compute_rhs as their RHS:
+ * out = c[0]*x[0] + c[1]*x[1] + ... + c[nterms-1]*x[nterms-1] over
+ * state-shaped (3, N0, N1, N2/2+1) complex arrays, 1 <= nterms <= 10.  The
+ * coefficients are rounded to the working precision, each product is rounded
+ * before it is added and the terms are added left to right (NumPy's order
+ * for `u = x0; u += c1*x1; ...`).  out may be one of the x (in place). */
+int sdns_lincomb(sdns_plan* plan, void* out, int nterms, const void* const* x, const double* c);
+/* Embedded error estimate of adaptiveRK with errnorm "2"
+ * (integrators.py:77-89): out[k] (device, FP64) = sum over the modes of
+ * component k of |err_k / sc_k|^2 with sc = atol + max(|u0|, |u1|)*rtol in
+ * the working precision; the caller takes max_k sqrt(out[k]) / sqrt(N0 N1 Nh). */
+int sdns_error_norm(sdns_plan* plan, const void* err, const void* u0, const void* u1, double atol, double rtol,
+                    double* out);
+
 /* Large-scale forcing (demo/Isotropic.py:154-184) in place on u_hat: zero the
  * mean mode, then scale the modes with |K|^2 <= kf^2 by
  * alpha = sqrt((target - E_upper) / E_lower) so that energy_fourier(u_hat)

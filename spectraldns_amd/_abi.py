@@ -38,6 +38,8 @@ _SIGS = {
     "sdns_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                       ctypes.c_double, _P, _P]),
     "sdns_energy": (ctypes.c_int, [_P, _P, ctypes.c_double, _P]),
+    "sdns_lincomb": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double)]),
+    "sdns_error_norm": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_force": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -271,6 +273,36 @@ class Plan:
                                         arr, n, float(nu), _ptr(diag) if diag is not None else None,
                                         _ptr(p_hat) if p_hat is not None else None))
         return u_hat
+
+    def lincomb(self, out, xs, cs):
+        """out = cs[0]*xs[0] + cs[1]*xs[1] + ... over state-shaped arrays (in place
+        when out is one of xs); coefficients rounded to the working precision."""
+        self._spec(out, 3, "out")
+        for x in xs:
+            self._spec(x, 3, "term")
+        n = len(xs)
+        if n != len(cs) or not 1 <= n <= 10:
+            raise SdnsError("lincomb takes 1..10 terms with one coefficient each")
+        ptrs = (_P * n)(*[x.data_ptr() for x in xs])
+        coef = (ctypes.c_double * n)(*[float(c) for c in cs])
+        self._sync_stream()
+        _check(self._lib.sdns_lincomb(self._h, _ptr(out), n, ptrs, coef))
+        return out
+
+    def error_norm(self, err, u0, u1, atol, rtol, out=None):
+        """Per component sum of |err/sc|^2, sc = atol + max(|u0|, |u1|)*rtol, as a
+        float64 GPU tensor of 3 entries."""
+        import torch
+        for t, nm in ((err, "err"), (u0, "u0"), (u1, "u1")):
+            self._spec(t, 3, nm)
+        if out is None:
+            out = torch.empty(3, dtype=torch.float64, device=err.device)
+        elif out.dtype != torch.float64 or out.numel() < 3 or out.device.type != "cuda":
+            raise SdnsError("out must be a float64 GPU tensor of 3 entries")
+        self._sync_stream()
+        _check(self._lib.sdns_error_norm(self._h, _ptr(err), _ptr(u0), _ptr(u1), float(atol), float(rtol),
+                                         _ptr(out)))
+        return out
 
     def energy(self, u_hat, nu, out):
         import torch

@@ -487,6 +487,27 @@ template <typename Rt> EwArgs<Rt> ew_args(sdns_plan* p) {
   return a;
 }
 
+template <typename Rt> int lincomb(sdns_plan* p, void* out, int nterms, const void* const* x, const double* c) {
+  LinArgs<Rt> a{};
+  a.out = out;
+  a.nterms = nterms;
+  a.n = 3L * p->N[0] * p->N[1] * p->Nh;
+  for (int j = 0; j < nterms; ++j) {
+    a.x[j] = x[j];
+    a.c[j] = (Rt)c[j];
+  }
+  if (launch_lincomb<Rt>(a, p->stream)) return fail(SDNS_ERR_INVALID, "nterms %d", nterms);
+  return 0;
+}
+
+template <typename Rt> int error_norm(sdns_plan* p, const void* err, const void* u0, const void* u1, double atol,
+                                      double rtol, double* out) {
+  const int nblk = 1024;  // 3 * nblk partials fit the partials buffer (>= 8192 doubles)
+  launch_errnorm<Rt>(err, u0, u1, (Rt)atol, (Rt)rtol, (long)p->N[0] * p->N[1] * p->Nh, p->part, nblk, p->stream);
+  if (launch_sum_rows(p->part, nblk, 3, out, p->stream)) return fail(SDNS_ERR_HIP, "sum rows");
+  return 0;
+}
+
 template <typename Rt> int energy(sdns_plan* p, const void* u, double nu, double* out) {
   EwArgs<Rt> a = ew_args<Rt>(p);
   a.a = u;
@@ -1021,6 +1042,25 @@ int sdns_energy(sdns_plan* p, const void* u, double nu, double* out) {
   int rc = ensure_workspace(p);
   if (rc) return rc;
   rc = p->precision == SDNS_DOUBLE ? energy<double>(p, u, nu, out) : energy<float>(p, u, nu, out);
+  return rc ? rc : check_launch();
+}
+
+int sdns_lincomb(sdns_plan* p, void* out, int nterms, const void* const* x, const double* c) {
+  if (!p || !out || !x || !c || nterms < 1 || nterms > SDNS_LIN_MAX)
+    return fail(SDNS_ERR_INVALID, "bad argument (1 <= nterms <= %d)", SDNS_LIN_MAX);
+  for (int j = 0; j < nterms; ++j)
+    if (!x[j]) return fail(SDNS_ERR_INVALID, "null term %d", j);
+  const int rc = p->precision == SDNS_DOUBLE ? lincomb<double>(p, out, nterms, x, c) : lincomb<float>(p, out, nterms, x, c);
+  return rc ? rc : check_launch();
+}
+
+int sdns_error_norm(sdns_plan* p, const void* err, const void* u0, const void* u1, double atol, double rtol,
+                    double* out) {
+  if (!p || !err || !u0 || !u1 || !out) return fail(SDNS_ERR_INVALID, "bad argument");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = p->precision == SDNS_DOUBLE ? error_norm<double>(p, err, u0, u1, atol, rtol, out)
+                                   : error_norm<float>(p, err, u0, u1, atol, rtol, out);
   return rc ? rc : check_launch();
 }
 

@@ -61,6 +61,25 @@ template <typename Rt> int launch_force_scale(const EwArgs<Rt>& a, Rt kf2, const
 template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* partials, int nblk, hipStream_t s);
 int launch_sum_rows(const double* partials, int nblk, int width, double* out, hipStream_t s);
 
+// ------------------------------------------------------------------ integrator updates
+// Linear combinations of state-shaped complex arrays, out = c[0] x[0] + c[1] x[1] + ...
+// (maths/integrators.py:66-75,126-139,161-175): each product is rounded before it is
+// added, the terms are added left to right; out may be one of the x.
+constexpr int SDNS_LIN_MAX = 10;
+template <typename Rt> struct LinArgs {
+  void* out;
+  const void* x[SDNS_LIN_MAX];
+  Rt c[SDNS_LIN_MAX];
+  int nterms;
+  long n;             // complex elements per array
+};
+template <typename Rt> int launch_lincomb(const LinArgs<Rt>& a, hipStream_t s);
+// Embedded-error norm of adaptiveRK (integrators.py:77-89): FP64 partial sums per block and
+// component of |err / sc|^2, sc = atol + max(|u0|, |u1|) rtol in the working precision;
+// layout [nblk][3].
+template <typename Rt> int launch_errnorm(const void* err, const void* u0, const void* u1, Rt atol, Rt rtol,
+                                          long n, double* partials, int nblk, hipStream_t s);
+
 // ------------------------------------------------------------------ batched ensembles
 // Line lengths of the batched pass kernels (strided x/y lines; half-length z
 // lines).  Every block of a batched launch has the same size whatever the

@@ -426,6 +426,63 @@ template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* 
   return 0;
 }
 
+// ------------------------------------------------------------------ integrator updates
+template <typename Rt> __global__ void __launch_bounds__(256) k_lincomb(LinArgs<Rt> a) {
+#pragma clang fp contract(off)
+  using C = cx<Rt>;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (long)gridDim.x * 256) {
+    const C x0 = ((const C*)a.x[0])[i];
+    C s = mk<C>(a.c[0] * x0.x, a.c[0] * x0.y);
+#pragma unroll
+    for (int j = 1; j < SDNS_LIN_MAX; ++j) {
+      if (j < a.nterms) {
+        const C xj = ((const C*)a.x[j])[i];
+        const Rt px = a.c[j] * xj.x, py = a.c[j] * xj.y;
+        s = mk<C>(s.x + px, s.y + py);
+      }
+    }
+    ((C*)a.out)[i] = s;
+  }
+}
+
+template <typename Rt> int launch_lincomb(const LinArgs<Rt>& a, hipStream_t s) {
+  if (a.nterms < 1 || a.nterms > SDNS_LIN_MAX) return -1;
+  const long nblk = std::min<long>((a.n + 255) / 256, 8192);
+  hipLaunchKernelGGL((k_lincomb<Rt>), dim3((unsigned)std::max<long>(nblk, 1)), dim3(256), 0, s, a);
+  return 0;
+}
+
+template <typename Rt>
+__global__ void __launch_bounds__(256) k_errnorm(const void* err, const void* u0, const void* u1, Rt atol, Rt rtol,
+                                                 long n, double* __restrict__ part) {
+#pragma clang fp contract(off)
+  using C = cx<Rt>;
+  const int comp = blockIdx.y;
+  const C* E = (const C*)err + comp * n;
+  const C* A = (const C*)u0 + comp * n;
+  const C* B = (const C*)u1 + comp * n;
+  double acc = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const C a = A[i], b = B[i], e = E[i];
+    const Rt ma = hypot(a.x, a.y), mb = hypot(b.x, b.y);
+    const Rt sc = atol + (ma > mb ? ma : mb) * rtol;
+    const Rt m = hypot(e.x / sc, e.y / sc);
+    acc += (double)(m * m);
+  }
+  __shared__ double red[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[3 * blockIdx.x + comp] = red[0] + red[1] + red[2] + red[3];
+}
+
+template <typename Rt> int launch_errnorm(const void* err, const void* u0, const void* u1, Rt atol, Rt rtol,
+                                          long n, double* partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL((k_errnorm<Rt>), dim3(nblk, 3), dim3(256), 0, s, err, u0, u1, atol, rtol, n, partials);
+  return 0;
+}
+
 }  // namespace sdns
 
 #define SDNS_INSTANTIATE(Rt)                                                                     \
@@ -441,4 +498,6 @@ template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* 
   template int launch_force_reduce<Rt>(const EwArgs<Rt>&, Rt, double*, int, hipStream_t);        \
   template int launch_force_scale<Rt>(const EwArgs<Rt>&, Rt, const double*, double, hipStream_t); \
   template int launch_spectrum<Rt>(const EwArgs<Rt>&, int, double*, int, hipStream_t);            \
+  template int launch_lincomb<Rt>(const LinArgs<Rt>&, hipStream_t);                               \
+  template int launch_errnorm<Rt>(const void*, const void*, const void*, Rt, Rt, long, double*, int, hipStream_t); \
   }

@@ -82,6 +82,13 @@ int sdns_cross2(sdns_plan* plan, void* c_hat, const void* a_hat);
 int sdns_add_pressure_diffusion(sdns_plan* plan, void* rhs, const void* u_hat, double nu, void* p_hat);
 int sdns_divergence(sdns_plan* plan, void* div_hat, const void* u_hat);
 int sdns_mul_minus_i(sdns_plan* plan, void* out, const void* a, size_t n);
+/* Velocity-vorticity solver (spectralDNS/solvers/VV.py):
+ *  cross2_k_over_k2: u_hat = 1j*((K/K^2) x w_hat), K^2 = 0 taken as 1
+ *    (compute_velocity, VV.py:57-73: cross2(v_hat, K_over_K2, w_hat))
+ *  add_linear: rhs -= nu*K2*w_hat, then rhs += source when not NULL
+ *    (add_linear, VV.py:108-113), in place on rhs */
+int sdns_cross2_k_over_k2(sdns_plan* plan, void* u_hat, const void* w_hat);
+int sdns_add_linear(sdns_plan* plan, void* rhs, const void* w_hat, double nu, const void* source);
 int sdns_mask_nyquist(sdns_plan* plan, void* a_hat);
 
 /* Conv of getConvection('Vortex') (NS.py:191-198): rhs = VTp.forward(u x curl u)

@@ -32,6 +32,8 @@ _SIGS = {
     "sdns_add_pressure_diffusion": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
     "sdns_divergence": (ctypes.c_int, [_P, _P, _P]),
     "sdns_mul_minus_i": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "sdns_cross2_k_over_k2": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_add_linear": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
     "sdns_mask_nyquist": (ctypes.c_int, [_P, _P]),
     "sdns_convection": (ctypes.c_int, [_P, _P, _P]),
     "sdns_compute_rhs": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
@@ -203,6 +205,25 @@ class Plan:
         self._sync_stream()
         _check(self._lib.sdns_cross2(self._h, _ptr(c_hat), _ptr(a_hat)))
         return c_hat
+
+    def cross2_k_over_k2(self, u_hat, w_hat):
+        """u_hat = 1j*((K/K^2) x w_hat), K^2 = 0 taken as 1 (solvers/VV.py:57-73)."""
+        self._spec(u_hat, 3, "u_hat")
+        self._spec(w_hat, 3, "w_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_cross2_k_over_k2(self._h, _ptr(u_hat), _ptr(w_hat)))
+        return u_hat
+
+    def add_linear(self, rhs, w_hat, nu, source=None):
+        """rhs -= nu*K2*w_hat; rhs += source (solvers/VV.py:110-115), in place."""
+        self._spec(rhs, 3, "rhs")
+        self._spec(w_hat, 3, "w_hat")
+        if source is not None:
+            self._spec(source, 3, "Source")
+        self._sync_stream()
+        _check(self._lib.sdns_add_linear(self._h, _ptr(rhs), _ptr(w_hat), float(nu),
+                                         _ptr(source) if source is not None else None))
+        return rhs
 
     def add_pressure_diffusion(self, rhs, u_hat, nu, p_hat=None):
         self._spec(rhs, 3, "rhs")

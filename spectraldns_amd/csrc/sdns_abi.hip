@@ -950,6 +950,14 @@ int sdns_cross2(sdns_plan* p, void* c, const void* a) {
   if (!a) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_CROSS2, c, a, nullptr, nullptr, 0, -1);
 }
+int sdns_cross2_k_over_k2(sdns_plan* p, void* c, const void* a) {
+  if (!a) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_CROSS2_KOK, c, a, nullptr, nullptr, 0, -1);
+}
+int sdns_add_linear(sdns_plan* p, void* rhs, const void* w_hat, double nu, const void* source) {
+  if (!w_hat) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_ADD_LINEAR, rhs, w_hat, source, nullptr, nu, -1);
+}
 int sdns_add_pressure_diffusion(sdns_plan* p, void* rhs, const void* u, double nu, void* ph) {
   if (!u) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_PRESSURE_DIFFUSION, rhs, u, nullptr, ph, nu, -1);

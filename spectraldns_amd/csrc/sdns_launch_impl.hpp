@@ -204,6 +204,35 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
       O[i] = cmuli<1>(c0);
       O[n + i] = cmuli<1>(c1);
       O[2 * n + i] = cmuli<1>(c2);
+    } else if constexpr (OP == EW_CROSS2_KOK) {  // c = 1j*((K/K^2) x a), K^2 = 0 -> 1 (solvers/VV.py:71-72)
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2 * K2;
+      const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+      const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2 / den;
+      const C a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
+      const C c0 = mk<C>(o1 * a2.x - o2 * a1.x, o1 * a2.y - o2 * a1.y);
+      const C c1 = mk<C>(o2 * a0.x - o0 * a2.x, o2 * a0.y - o0 * a2.y);
+      const C c2 = mk<C>(o0 * a1.x - o1 * a0.x, o0 * a1.y - o1 * a0.y);
+      O[i] = cmuli<1>(c0);
+      O[n + i] = cmuli<1>(c1);
+      O[2 * n + i] = cmuli<1>(c2);
+    } else if constexpr (OP == EW_ADD_LINEAR) {  // rhs -= nu*K2*w_hat; rhs += Source (VV.py:108-113)
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2 * K2;
+      const Rt z = a.nu * k2;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const C w = A[c * n + i];
+        C r = O[c * n + i];
+        r = mk<C>(r.x - z * w.x, r.y - z * w.y);
+        if (a.b) {
+          const C src = ((const C*)a.b)[c * n + i];
+          r = mk<C>(r.x + src.x, r.y + src.y);
+        }
+        O[c * n + i] = r;
+      }
     } else if constexpr (OP == EW_DIVERGENCE) {  // 1j*(K . a) (NS.py:106-110)
       const C a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
       const C d = mk<C>(K0 * a0.x + K1 * a1.x + K2 * a2.x, K0 * a0.y + K1 * a1.y + K2 * a2.y);
@@ -267,6 +296,8 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
     case EW_MUL_MINUS_I: hipLaunchKernelGGL((k_elementwise<Rt, EW_MUL_MINUS_I>), g, b, 0, s, a); return 0;
     case EW_MASK_NYQUIST: hipLaunchKernelGGL((k_elementwise<Rt, EW_MASK_NYQUIST>), g, b, 0, s, a); return 0;
     case EW_SGS_FORCE: hipLaunchKernelGGL((k_elementwise<Rt, EW_SGS_FORCE>), g, b, 0, s, a); return 0;
+    case EW_CROSS2_KOK: hipLaunchKernelGGL((k_elementwise<Rt, EW_CROSS2_KOK>), g, b, 0, s, a); return 0;
+    case EW_ADD_LINEAR: hipLaunchKernelGGL((k_elementwise<Rt, EW_ADD_LINEAR>), g, b, 0, s, a); return 0;
     default: return -1;
   }
 }

@@ -18,7 +18,7 @@ import numpy as np
 
 from ..optimization import wraps
 
-__all__ = ["getintegrator", "RK4", "ForwardEuler", "AB2", "adaptiveRK"]
+__all__ = ["getintegrator", "RK4", "RK4_stages", "ForwardEuler", "AB2", "adaptiveRK"]
 
 # Bogacki-Shampine 5(4): the published coefficients (integrators.py:200-209)
 _BS5_A = [[0, 0, 0, 0, 0, 0, 0, 0],
@@ -42,6 +42,23 @@ def RK4(u0, u1, u2, rhs, a, b, dt, solver, context):
     plan = context.VT.plan
     lo = context.get("U_hat_lo") if solver.params.get("compensated", False) else None
     plan.rk4_steps(u0, [dt], solver.params.nu, p_hat=context.P_hat, u_lo=lo)
+    return u0, dt, dt
+
+
+def RK4_stages(u0, u1, u2, rhs, a, b, dt, solver, context):
+    """RK4 stage by stage (integrators.py:149-159) for a ComputeRHS other than
+    the fused NS one (the VV solver, or a replaced ComputeRHS): four
+    solver.ComputeRHS calls with the updates u0 = u1 + (b dt) rhs and
+    u2 += (a dt) rhs as sdns_lincomb launches."""
+    plan = context.VT.plan
+    plan.lincomb(u1, [u0], [1.0])
+    plan.lincomb(u2, [u0], [1.0])
+    for rk in range(4):
+        rhs = solver.ComputeRHS(rhs, u0, solver, **context)
+        if rk < 3:
+            plan.lincomb(u0, [u1, rhs], [1.0, b[rk] * dt])
+        plan.lincomb(u2, [u2, rhs], [1.0, a[rk] * dt])
+    plan.lincomb(u0, [u2], [1.0])
     return u0, dt, dt
 
 
@@ -119,11 +136,20 @@ def adaptiveRK(A, b, bhat, err_order, fY_hat, u0_new, sc, err, fsal, offset, aTO
 def getintegrator(rhs, u0, solver, context):
     """Return the integrator chosen by params.integrator (integrators.py:177-239)."""
     params = solver.params
+    if params.integrator == "RK4" and not (getattr(solver.conv, "fused_vortex", False) and
+                                           solver.ComputeRHS is getattr(solver, "_ComputeRHS_fused", None)):
+        if params.get("compensated", False):
+            raise NotImplementedError("the compensated state is carried by the fused NS RK4 step only")
+        a = np.array([1. / 6., 1. / 3., 1. / 3., 1. / 6.], dtype=context.float)
+        b = np.array([0.5, 0.5, 1.], dtype=context.float)
+        u1, u2 = u0.clone(), u0.clone()
+
+        @wraps(RK4_stages)
+        def func():
+            return RK4_stages(u0, u1, u2, rhs, a, b, params.dt, solver, context)
+        return func
+
     if params.integrator == "RK4":
-        if not getattr(solver.conv, "fused_vortex", False):
-            raise NotImplementedError("the fused RK4 step serves the Vortex convection of NS")
-        if solver.ComputeRHS is not getattr(solver, "_ComputeRHS_fused", None):
-            raise NotImplementedError("a replaced ComputeRHS cannot run inside the fused RK4 step")
         if params.get("compensated", False) and params.precision == "single" and "U_hat_lo" not in context:
             import torch
             context["U_hat_lo"] = torch.zeros_like(u0)

@@ -6,7 +6,7 @@ import torch
 
 from .._abi import Plan, SdnsError
 
-__all__ = ["cross1", "cross2", "add_pressure_diffusion_NS"]
+__all__ = ["cross1", "cross2", "add_pressure_diffusion_NS", "add_linear"]
 
 _util_plans = {}
 
@@ -25,11 +25,22 @@ def cross1(c, a, b):
 
 
 def cross2(c, a, b):
-    """c = 1j*(a x b) with a the wavenumber list of a plan (maths/cross.py:27-32)."""
+    """c = 1j*(a x b) with a the wavenumber list K of a plan, or the K_over_K2
+    handle of a velocity-vorticity context (maths/cross.py:27-32)."""
     plan = getattr(a, "plan", None)
     if plan is None:
-        raise SdnsError("cross2 needs the wavenumber list K of a context (it carries the plan)")
+        raise SdnsError("cross2 needs the wavenumber list K (or K_over_K2) of a context (it carries the plan)")
+    if getattr(a, "k_over_k2", False):
+        return plan.cross2_k_over_k2(c, b)
     return plan.cross2(c, b)
+
+
+def add_linear(rhs, w_hat, nu, K2, Source):
+    """solvers/VV.py:110-115: rhs -= nu*K2*w_hat; rhs += Source."""
+    plan = getattr(K2, "plan", None)
+    if plan is None:
+        raise SdnsError("add_linear needs the K2 handle of a context (it carries the plan)")
+    return plan.add_linear(rhs, w_hat, nu, Source)
 
 
 def add_pressure_diffusion_NS(rhs, u_hat, nu, K2, K, P_hat, K_over_K2):

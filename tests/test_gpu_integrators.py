@@ -100,7 +100,7 @@ def test_integrators_fp32_vs_fp64_oracle(integrator):
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_lincomb_and_error_norm_kernels(precision):
     from spectraldns_amd._abi import Plan
-    N = (16, 12, 20)
+    N = (16, 12, 16)
     p = Plan(N, (TWO_PI,) * 3, precision, "2/3-rule", True, 0)
     fl = np.float64 if precision == "double" else np.float32
     rng = np.random.default_rng(4)

@@ -1,0 +1,113 @@
+"""Velocity-vorticity solver (spectralDNS/solvers/VV.py) on the GPU, driven
+through the reference's API as tests/test_NSVV.py:36-51 drives it (VV on the
+uniform and the non-uniform box, 2/3 and 3/2 rule, Taylor-Green pins at
+ntol 7), and compared with the VV oracle (oracle/vv.py) on the same inputs:
+FP64 within 1e-12 relative L2 after the ten RK4 steps; FP32 within 1e-5 of
+the FP64 oracle.  The two VV kernels against their NumPy definitions."""
+import numpy as np
+import pytest
+
+from oracle import vv as OV
+from oracle.ns import Grid, dt_schedule
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+TWO_PI = 2 * np.pi
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def initialize2(solver, context):
+    """tests/TG.py:29-35: TG velocity, W_hat = cross2(K, U_hat)."""
+    from spectraldns_amd import config
+    U, X = context.U, context.X
+    U[0] = torch.sin(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+    U[1] = -torch.cos(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+    U[2] = 0
+    solver.set_velocity(**context)
+    solver.cross2(context.W_hat, context.K, context.U_hat)
+    config.params.t = 0.0
+    config.params.tstep = 0
+
+
+def run_vv(args, dealias, precision="double", ntol=7):
+    from spectraldns_amd import config, get_solver, solve
+    print("\n[case] VV %s %s %s" % (" ".join(args), dealias, precision), flush=True)
+    seen = {}
+
+    def regression(context):
+        U = solver.get_velocity(**context)
+        curl = solver.get_curl(**context)
+        n = np.prod(config.params.N)
+        seen["w"] = float(torch.sum(curl.double() * curl.double()).item()) / n / 2
+        seen["k"] = float(torch.sum(U.double() * U.double()).item()) / n / 2
+        assert round(seen["w"] - 0.375249930801, ntol) == 0, seen
+        assert round(seen["k"] - 0.124953117517, ntol) == 0, seen
+
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "convection": "Vortex", "dealias": dealias,
+                   "precision": precision})
+    solver = get_solver(regression_test=regression, parse_args=args + ["VV"])
+    context = solver.get_context()
+    assert "P" not in context and context.u is context.W_hat
+    initialize2(solver, context)
+    solve(solver, context)
+    w = context.W_hat.cpu().numpy().astype(np.complex128)
+    div = solver.get_divergence(**context)
+    config.update({"precision": "double"})
+    return w, seen, float(div.abs().max())
+
+
+CASES = [["--M", "4", "4", "4", "--L", "2*pi", "2*pi", "2*pi"],
+         ["--M", "6", "5", "4", "--L", "6*pi", "4*pi", "2*pi"]]
+
+
+@pytest.mark.parametrize("dealias", ["2/3-rule", "3/2-rule"])
+@pytest.mark.parametrize("args", CASES)
+def test_vv_taylor_green_fp64(args, dealias):
+    w, seen, div = run_vv(args, dealias)
+    M = [int(a) for a in args[1:4]]
+    L = (TWO_PI * 3, TWO_PI * 2, TWO_PI) if M[0] == 6 else (TWO_PI,) * 3
+    g = Grid(tuple(2 ** m for m in M), L, "double", dealias)
+    ref = OV.taylor_green(g)
+    for dt in dt_schedule(0.1, 0.01):
+        ref = OV.rk4_step(g, ref, dt, 0.000625)
+    assert rel(w, ref) < 1e-12, rel(w, ref)
+    assert div < 1e-12
+
+
+def test_vv_taylor_green_fp32():
+    args = ["--M", "5", "5", "5", "--L", "2*pi", "2*pi", "2*pi"]
+    w, seen, _ = run_vv(args, "2/3-rule", "single", ntol=5)
+    g = Grid((32, 32, 32), (TWO_PI,) * 3, "double", "2/3-rule")
+    ref = OV.taylor_green(g)
+    for dt in dt_schedule(0.1, 0.01, precision="single"):
+        ref = OV.rk4_step(g, ref, dt, 0.000625)
+    assert rel(w, ref) < 1e-5, rel(w, ref)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_vv_kernels(precision):
+    from spectraldns_amd._abi import Plan
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    N, L = (16, 12, 16), (TWO_PI, 3.0, 5.0)
+    p = Plan(N, L, precision, "2/3-rule", True, 0)
+    g = Grid(N, L, "double", "2/3-rule")
+    cd = torch.complex128 if precision == "double" else torch.complex64
+    u = random_solenoidal_field(N, L, seed=8, k0=3.0, energy=0.5)
+    w = g.cross2(u)
+    wt = torch.from_numpy(w).to(cd).cuda()
+    out = torch.empty_like(wt)
+    p.cross2_k_over_k2(out, wt)
+    tol = 1e-15 if precision == "double" else 1e-6
+    assert rel(out.cpu().numpy(), OV.cross2_k_over_k2(g, w)) < tol
+    # u_hat = 1j*(K/K^2) x (1j K x u_hat) for the solenoidal u without a mean mode
+    u0 = u.copy()
+    u0[:, 0, 0, 0] = 0
+    assert rel(out.cpu().numpy(), u0) < 1e3 * tol
+    rhs = torch.from_numpy(u).to(cd).cuda()
+    src = torch.from_numpy(0.5 * w).to(cd).cuda()
+    p.add_linear(rhs, wt, 0.02, src)
+    assert rel(rhs.cpu().numpy(), u - 0.02 * g.K2 * w + 0.5 * w) < 10 * tol

@@ -89,6 +89,21 @@ int sdns_mul_minus_i(sdns_plan* plan, void* out, const void* a, size_t n);
  *    (add_linear, VV.py:108-113), in place on rhs */
 int sdns_cross2_k_over_k2(sdns_plan* plan, void* u_hat, const void* w_hat);
 int sdns_add_linear(sdns_plan* plan, void* rhs, const void* w_hat, double nu, const void* source);
+/* MHD solver in Elsasser form (spectralDNS/solvers/MHD.py), 2/3 rule or no
+ * dealiasing:
+ *  dealias_truncate: out = a with the modes outside the plan's kept input
+ *    range zeroed, 3 components (the input truncation of shenfun's dealiased
+ *    backward transform with dealias_direct, as VMp.backward(ub_hat) in
+ *    Conv, MHD.py:140-146); SDNS_ERR_UNSUPPORTED for a 3/2-rule plan
+ *  elsasser_products: zz[3i+j] = (u_i + b_i)*(u_j - b_j) on npoints physical
+ *    points per component, ub = (u0, u1, u2, b0, b1, b2) (divergenceConvection,
+ *    MHD.py:118-130, before its forward transforms)
+ *  set_elsasser: c_hat[0:3] = -1j*sum_j K_j (zz_ij + zz_ji)/2 and
+ *    c_hat[3:6] = 1j*sum_j K_j (zz_ji - zz_ij)/2 from the 9 spectral
+ *    products (set_Elsasser, MHD.py:104-113) */
+int sdns_dealias_truncate(sdns_plan* plan, void* out, const void* a);
+int sdns_elsasser_products(sdns_plan* plan, void* zz, const void* ub, size_t npoints);
+int sdns_set_elsasser(sdns_plan* plan, void* c_hat, const void* zz_hat);
 int sdns_mask_nyquist(sdns_plan* plan, void* a_hat);
 
 /* Conv of getConvection('Vortex') (NS.py:191-198): rhs = VTp.forward(u x curl u)

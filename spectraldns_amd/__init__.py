@@ -28,8 +28,8 @@ def get_solver(update=None, regression_test=None, additional_callback=None,
     args = getattr(config, mesh).parse_args(parse_args)
     config.params.update(vars(args))
     name = config.params.solver
-    if name not in ("NS", "VV"):
-        raise NotImplementedError("solver %r is not built; the HIP path serves NS and VV" % name)
+    if name not in ("NS", "VV", "MHD"):
+        raise NotImplementedError("solver %r is not built; the HIP path serves NS, VV and MHD" % name)
     solver = importlib.import_module("spectraldns_amd.solvers." + name)
     if update:
         solver.update = update

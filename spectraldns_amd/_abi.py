@@ -34,6 +34,9 @@ _SIGS = {
     "sdns_mul_minus_i": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_cross2_k_over_k2": (ctypes.c_int, [_P, _P, _P]),
     "sdns_add_linear": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
+    "sdns_dealias_truncate": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_elsasser_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "sdns_set_elsasser": (ctypes.c_int, [_P, _P, _P]),
     "sdns_mask_nyquist": (ctypes.c_int, [_P, _P]),
     "sdns_convection": (ctypes.c_int, [_P, _P, _P]),
     "sdns_compute_rhs": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
@@ -214,6 +217,36 @@ class Plan:
         _check(self._lib.sdns_cross2_k_over_k2(self._h, _ptr(u_hat), _ptr(w_hat)))
         return u_hat
 
+    def dealias_truncate(self, out, a_hat):
+        """out = a_hat with the modes outside the kept input range zeroed (3 components)."""
+        self._spec(out, 3, "out")
+        self._spec(a_hat, 3, "a_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_dealias_truncate(self._h, _ptr(out), _ptr(a_hat)))
+        return out
+
+    def elsasser_products(self, zz, ub):
+        """zz[3i+j] = (u_i + b_i)(u_j - b_j) on physical (6, ...) -> (9, ...) arrays."""
+        for t, nm, nc in ((zz, "zz", 9), (ub, "ub", 6)):
+            if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous() \
+                    or tuple(t.shape) != (nc,) + self.physical_shape:
+                raise SdnsError("%s must be a contiguous (%d,) + %s %s GPU tensor"
+                                % (nm, nc, self.physical_shape, self.real_dtype))
+        self._sync_stream()
+        _check(self._lib.sdns_elsasser_products(self._h, _ptr(zz), _ptr(ub), ub[0].numel()))
+        return zz
+
+    def set_elsasser(self, c_hat, zz_hat):
+        """c_hat (6 components) from the 9 spectral products zz_hat (MHD.py:104-113)."""
+        if tuple(c_hat.shape) != (6,) + self.spectral_shape or tuple(zz_hat.shape) != (9,) + self.spectral_shape:
+            raise SdnsError("set_elsasser: c_hat (6, ...) and zz_hat (9, ...) spectral arrays required")
+        for t in (c_hat, zz_hat):
+            if t.device.type != "cuda" or t.dtype != self.complex_dtype or not t.is_contiguous():
+                raise SdnsError("set_elsasser: contiguous %s GPU tensors required" % self.complex_dtype)
+        self._sync_stream()
+        _check(self._lib.sdns_set_elsasser(self._h, _ptr(c_hat), _ptr(zz_hat)))
+        return c_hat
+
     def add_linear(self, rhs, w_hat, nu, source=None):
         """rhs -= nu*K2*w_hat; rhs += source (solvers/VV.py:110-115), in place."""
         self._spec(rhs, 3, "rhs")
@@ -297,7 +330,12 @@ class Plan:
 
     def lincomb(self, out, xs, cs):
         """out = cs[0]*xs[0] + cs[1]*xs[1] + ... over state-shaped arrays (in place
-        when out is one of xs); coefficients rounded to the working precision."""
+        when out is one of xs); coefficients rounded to the working precision.
+        Arrays of 3k components run as k launches of 3."""
+        if out.dim() == 4 and out.shape[0] > 3 and out.shape[0] % 3 == 0:
+            for c in range(0, out.shape[0], 3):
+                self.lincomb(out[c:c + 3], [x[c:c + 3] for x in xs], cs)
+            return out
         self._spec(out, 3, "out")
         for x in xs:
             self._spec(x, 3, "term")

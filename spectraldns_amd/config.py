@@ -170,11 +170,11 @@ triplyperiodic.add_argument("--M", default=[6, 6, 6], metavar=("Mx", "My", "Mz")
 triplyperiodic.add_argument("--TOL", type=float, default=1e-6, help="Tolerance for adaptive integrators")
 triplyperiodic.add_argument("--integrator", default="RK4",
                             choices=("RK4", "ForwardEuler", "AB2", "BS5_adaptive", "BS5_fixed"),
-                            help="Integrator (the HIP path implements RK4)")
+                            help="Integrator")
 _sub = triplyperiodic.add_subparsers(dest="solver")
 _sub.add_parser("NS", help="Regular Navier Stokes solver")
-_sub.add_parser("VV", help="Velocity-Vorticity formulation (not built)")
-_mhd = _sub.add_parser("MHD", help="Magnetohydrodynamics solver (not built)")
+_sub.add_parser("VV", help="Velocity-Vorticity formulation")
+_mhd = _sub.add_parser("MHD", help="Magnetohydrodynamics solver (2/3 rule or no dealiasing)")
 _mhd.add_argument("--eta", default=0.01, type=float, help="MHD parameter")
 _bq = _sub.add_parser("Bq", help="Boussinesq (not built)")
 _bq.add_argument("--Ri", default=0.1, type=float, help="Richardson number")

@@ -958,6 +958,19 @@ int sdns_add_linear(sdns_plan* p, void* rhs, const void* w_hat, double nu, const
   if (!w_hat) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_ADD_LINEAR, rhs, w_hat, source, nullptr, nu, -1);
 }
+int sdns_dealias_truncate(sdns_plan* p, void* out, const void* a) {
+  if (!a) return fail(SDNS_ERR_INVALID, "null input");
+  if (p && p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
+  return ew(p, EW_TRUNCATE, out, a, nullptr, nullptr, 0, -1);
+}
+int sdns_set_elsasser(sdns_plan* p, void* c_hat, const void* zz_hat) {
+  if (!zz_hat) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_SET_ELSASSER, c_hat, zz_hat, nullptr, nullptr, 0, -1);
+}
+int sdns_elsasser_products(sdns_plan* p, void* zz, const void* ub, size_t npoints) {
+  if (!ub) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_ELSASSER_PRODUCTS, zz, ub, nullptr, nullptr, 0, (long)npoints);
+}
 int sdns_add_pressure_diffusion(sdns_plan* p, void* rhs, const void* u, double nu, void* ph) {
   if (!u) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_PRESSURE_DIFFUSION, rhs, u, nullptr, ph, nu, -1);

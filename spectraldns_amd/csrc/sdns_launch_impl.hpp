@@ -184,6 +184,21 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
     O[n + i] = a2 * b0 - a0 * b2;
     O[2 * n + i] = a0 * b1 - a1 * b0;
     return;
+  } else if constexpr (OP == EW_ELSASSER_PRODUCTS) {  // solvers/MHD.py:118-130 on n physical points
+    const Rt* A = (const Rt*)a.a;  // u0 u1 u2 b0 b1 b2
+    Rt* O = (Rt*)a.out;            // z0_i * z1_j at field 3 i + j
+    Rt z0[3], z1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const Rt u = A[c * n + i], bb = A[(3 + c) * n + i];
+      z0[c] = u + bb;
+      z1[c] = u - bb;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) O[(3 * r + c) * n + i] = z0[r] * z1[c];
+    return;
   } else if constexpr (OP == EW_MUL_MINUS_I) {  // out = -1j * a (NS.py:96-99)
     const C v = ((const C*)a.a)[i];
     ((C*)a.out)[i] = mk<C>(v.y, -v.x);
@@ -232,6 +247,27 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
           r = mk<C>(r.x + src.x, r.y + src.y);
         }
         O[c * n + i] = r;
+      }
+    } else if constexpr (OP == EW_TRUNCATE) {  // 2/3-rule input truncation of a dealiased backward
+      const int kx = ix < a.N0 / 2 ? ix : ix - a.N0, ky = iy < a.N1 / 2 ? iy : iy - a.N1;
+      const bool keep = (kx < 0 ? -kx : kx) <= a.keep[0] && (ky < 0 ? -ky : ky) <= a.keep[1] && kz <= a.keep[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) O[c * n + i] = keep ? A[c * n + i] : czero<C>();
+    } else if constexpr (OP == EW_SET_ELSASSER) {  // solvers/MHD.py:104-113, a = ZZ_hat[3][3], out = c[6]
+      const Rt Kv[3] = {K0, K1, K2};
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        C sp = czero<C>(), sm = czero<C>();
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const C zrj = A[(3 * r + j) * n + i], zjr = A[(3 * j + r) * n + i];
+          const C p = mk<C>(Kv[j] * (zrj.x + zjr.x), Kv[j] * (zrj.y + zjr.y));
+          const C m = mk<C>(Kv[j] * (zjr.x - zrj.x), Kv[j] * (zjr.y - zrj.y));
+          sp = j == 0 ? p : mk<C>(sp.x + p.x, sp.y + p.y);
+          sm = j == 0 ? m : mk<C>(sm.x + m.x, sm.y + m.y);
+        }
+        O[r * n + i] = mk<C>(sp.y / Rt(2), -sp.x / Rt(2));        // -1j*s/2
+        O[(3 + r) * n + i] = mk<C>(-sm.y / Rt(2), sm.x / Rt(2));  // 1j*s/2
       }
     } else if constexpr (OP == EW_DIVERGENCE) {  // 1j*(K . a) (NS.py:106-110)
       const C a0 = A[i], a1 = A[n + i], a2 = A[2 * n + i];
@@ -298,6 +334,9 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
     case EW_SGS_FORCE: hipLaunchKernelGGL((k_elementwise<Rt, EW_SGS_FORCE>), g, b, 0, s, a); return 0;
     case EW_CROSS2_KOK: hipLaunchKernelGGL((k_elementwise<Rt, EW_CROSS2_KOK>), g, b, 0, s, a); return 0;
     case EW_ADD_LINEAR: hipLaunchKernelGGL((k_elementwise<Rt, EW_ADD_LINEAR>), g, b, 0, s, a); return 0;
+    case EW_TRUNCATE: hipLaunchKernelGGL((k_elementwise<Rt, EW_TRUNCATE>), g, b, 0, s, a); return 0;
+    case EW_SET_ELSASSER: hipLaunchKernelGGL((k_elementwise<Rt, EW_SET_ELSASSER>), g, b, 0, s, a); return 0;
+    case EW_ELSASSER_PRODUCTS: hipLaunchKernelGGL((k_elementwise<Rt, EW_ELSASSER_PRODUCTS>), g, b, 0, s, a); return 0;
     default: return -1;
   }
 }

@@ -1,0 +1,97 @@
+"""MHD in Elsasser form (spectralDNS/solvers/MHD.py) on the GPU, driven through
+the reference's API as tests/test_MHD.py:32-49 drives it (Taylor-Green U and
+B of tests/TGMHD.py:4-14 on the uniform and the non-uniform box, nu =
+0.000625, eta = 0.01, Divergence convection, RK4, 2/3 rule; pins k(0.1) =
+0.124565408177 and b(0.1) = 0.124637762143 at ntol 7), and compared with the
+MHD oracle (oracle/mhd.py): FP64 within 1e-12 relative L2 after the ten
+steps.  The three MHD kernels against their NumPy definitions."""
+import numpy as np
+import pytest
+
+from oracle import mhd as OM
+from oracle.ns import Grid, dt_schedule
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+TWO_PI = 2 * np.pi
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+CASES = [["--M", "4", "4", "4", "--L", "2*pi", "2*pi", "2*pi"],
+         ["--M", "6", "5", "4", "--L", "6*pi", "4*pi", "2*pi"]]
+
+
+@pytest.mark.parametrize("args", CASES)
+def test_mhd_taylor_green_fp64(args):
+    from spectraldns_amd import config, get_solver, solve
+    print("\n[case] MHD %s" % " ".join(args), flush=True)
+    seen = {}
+
+    def regression(context):
+        UB = context.VM.backward(context.UB_hat, context.UB)
+        n = np.prod(config.params.N)
+        seen["k"] = float(torch.sum(UB[:3].double() ** 2).item()) / n / 2
+        seen["b"] = float(torch.sum(UB[3:].double() ** 2).item()) / n / 2
+        assert round(seen["k"] - 0.124565408177, 7) == 0, seen
+        assert round(seen["b"] - 0.124637762143, 7) == 0, seen
+
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "eta": 0.01, "convection": "Divergence",
+                   "dealias": "2/3-rule", "precision": "double"})
+    solver = get_solver(regression_test=regression, parse_args=args + ["MHD"])
+    c = solver.get_context()
+    X = c.X
+    c.U[0] = torch.sin(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+    c.U[1] = -torch.cos(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+    c.U[2] = 0
+    c.B[0] = torch.sin(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+    c.B[1] = torch.cos(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+    c.B[2] = 0
+    c.VM.forward(c.UB, c.UB_hat)
+    config.params.t = 0.0
+    config.params.tstep = 0
+    solve(solver, c)
+    ub = c.UB_hat.cpu().numpy()
+    div = float(solver.get_divergence(**c).abs().max())
+    config.update({"convection": "Vortex"})
+    M = [int(a) for a in args[1:4]]
+    L = (TWO_PI * 3, TWO_PI * 2, TWO_PI) if M[0] == 6 else (TWO_PI,) * 3
+    g = Grid(tuple(2 ** m for m in M), L, "double", "2/3-rule")
+    ref = OM.taylor_green(g)
+    for dt in dt_schedule(0.1, 0.01):
+        ref = OM.rk4_step(g, ref, dt, 0.000625, 0.01)
+    assert rel(ub, ref) < 1e-12, rel(ub, ref)
+    assert div < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_mhd_kernels(precision):
+    from spectraldns_amd._abi import Plan
+    N, L = (16, 12, 16), (TWO_PI, 3.0, 5.0)
+    p = Plan(N, L, precision, "2/3-rule", True, 0)
+    g = Grid(N, L, "double", "2/3-rule")
+    rd = np.float64 if precision == "double" else np.float32
+    cd = np.complex128 if precision == "double" else np.complex64
+    tol = 1e-15 if precision == "double" else 1e-6
+    rng = np.random.default_rng(6)
+    ub = rng.standard_normal((6,) + N).astype(rd)
+    zz = torch.empty((9,) + N, dtype=torch.float64 if precision == "double" else torch.float32, device="cuda")
+    p.elsasser_products(zz, torch.from_numpy(ub).cuda())
+    z0, z1 = ub[:3] + ub[3:], ub[:3] - ub[3:]
+    expect = np.stack([z0[i] * z1[j] for i in range(3) for j in range(3)])
+    assert rel(zz.cpu().numpy(), expect) < tol
+    shp = (9,) + g.K2.shape
+    ZZ = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(cd)
+    c = torch.empty((6,) + g.K2.shape, dtype=torch.complex128 if precision == "double" else torch.complex64,
+                    device="cuda")
+    p.set_elsasser(c, torch.from_numpy(ZZ).cuda())
+    assert rel(c.cpu().numpy(), OM.set_elsasser(g, ZZ.astype(np.complex128).reshape((3, 3) + g.K2.shape))) < 10 * tol
+    a = torch.from_numpy(ZZ[:3]).cuda()
+    out = torch.empty_like(a)
+    p.dealias_truncate(out, a)
+    kx, ky, kz = (np.abs(k) for k in g.kint)
+    keep = (3 * kx[:, None, None] < N[0]) & (3 * ky[None, :, None] < N[1]) & (3 * kz[None, None, :] < N[2])
+    assert np.array_equal(out.cpu().numpy(), np.where(keep, ZZ[:3], 0))

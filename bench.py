@@ -211,6 +211,90 @@ def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
     return time.perf_counter() - t0, _WORKERS
 
 
+# Algorithmic HBM bytes per time step of the solver workloads, in units of
+# F = N0*N1*(N2/2+1)*esz (one complex spectral field; a real physical field is
+# counted as F too), 2/3 rule.  VV RHS: K/K^2 curl-inverse 6F, fused
+# convection passes 40F (P1-P4 34F + convection-only x-forward 6F), cross2 6F,
+# add_linear 9F = 61F; RK4 stage by stage: two state copies 12F, three
+# u0 = u1 + c rhs 27F, four u2 += c rhs 36F, final copy 6F = 81F.  MHD RHS:
+# truncation 12F, six inverse transforms 36F (x, y, z pass: 6F per field),
+# Elsasser products 15F, nine forward transforms 54F, set_Elsasser 15F,
+# pressure/diffusion of U 10F, diffusion of B 9F = 151F; RK4 over six
+# components 162F.  NS integrators: fused RHS 44F (P1-P4 34F; x-forward with
+# the projection reads W4 and u_hat, writes rhs and P_hat: 10F);
+# ForwardEuler update 9F; AB2 updates 12F + 6F; BS5 (FSAL: 7 RHS per step
+# after the first) stage sums sum_{i=1..7} 3(i+2)F = 126F, solution sum 30F,
+# error sum 27F, error norm 9F, copy of the new state 6F.
+SOLVER_BYTES_F = {
+    ("VV", "RK4"): 4 * 61 + 81,
+    ("MHD", "RK4"): 4 * 151 + 162,
+    ("NS", "ForwardEuler"): 44 + 9,
+    ("NS", "AB2"): 44 + 18,
+    ("NS", "BS5_fixed"): 7 * 44 + 126 + 30 + 27 + 9 + 6,
+}
+
+
+def solver_main(args):
+    """One time step of a solver/integrator pair through the reference's API
+    (get_solver, get_context, getintegrator), Taylor-Green start, 2/3 rule."""
+    import torch
+    from spectraldns_amd import config, get_solver
+    m = int(round(np.log2(args.N)))
+    conv = "Divergence" if args.solver == "MHD" else "Vortex"
+    config.update({"nu": 0.000625, "dt": 0.001, "T": 1e9, "eta": 0.01, "convection": conv,
+                   "dealias": "2/3-rule", "precision": args.precision, "integrator": args.integrator})
+    solver = get_solver(parse_args=["--M", str(m), str(m), str(m), args.solver])
+    c = solver.get_context()
+    X = c.X
+    c.U[0] = torch.sin(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+    c.U[1] = -torch.cos(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+    c.U[2] = 0
+    if args.solver == "MHD":
+        c.B[0] = torch.sin(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+        c.B[1] = torch.cos(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+        c.VM.forward(c.UB, c.UB_hat)
+    else:
+        solver.set_velocity(**c)
+        if args.solver == "VV":
+            solver.cross2(c.W_hat, c.K, c.U_hat)
+    config.params.t = 0.0
+    config.params.tstep = 0
+    solver.conv = solver.getConvection(conv)
+    integrate = solver.getintegrator(c.dU, c.u, solver, c)
+
+    def run(n):
+        for _ in range(n):
+            integrate()
+            config.params.tstep += 1
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sps = args.steps / el
+    esz = 8 if args.precision == "single" else 16
+    F = args.N * args.N * (args.N // 2 + 1) * esz
+    nf = SOLVER_BYTES_F.get((args.solver, args.integrator))
+    out = {
+        "metric": "%s %s time steps/s at %d^3" % (args.solver, args.integrator, args.N),
+        "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000 * el / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "single" else "f64",
+        "data": "synthetic (Taylor-Green field)",
+        "config": {"workload": "%s solver, %s, %d^3, 2/3 rule, nu=0.000625, dt=0.001%s"
+                               % (args.solver, args.integrator, args.N, ", eta=0.01" if args.solver == "MHD" else ""),
+                   "N": args.N},
+    }
+    if nf is not None:
+        out["step_roofline"] = {"algorithmic_bytes_per_step": nf * F, "F_units": nf,
+                                "achieved_GBps": round(nf * F * sps / 1e9, 1),
+                                "frac": round(nf * F * sps / 1e9 / HBM_PEAK_GBS, 4)}
+    config.update({"convection": "Vortex", "integrator": "RK4", "precision": "double"})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,9 +304,13 @@ def main():
     ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble", "forced"),
+    ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble", "forced", "solver"),
                     help="isotropic: the 256^3 box (default); ensemble: the 96-box uneven ensemble; "
-                         "forced: forced isotropic turbulence with compensated FP32 (use --N 1024)")
+                         "forced: forced isotropic turbulence with compensated FP32 (use --N 1024); "
+                         "solver: time steps of --solver with --integrator through the solver API")
+    ap.add_argument("--solver", default="VV", choices=("NS", "VV", "MHD"))
+    ap.add_argument("--integrator", default="RK4", choices=("RK4", "ForwardEuler", "AB2", "BS5_fixed"))
+    ap.add_argument("--precision", default="single", choices=("single", "double"))
     ap.add_argument("--no-graph", action="store_true", help="ensemble: eager launches instead of graph replay")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first copy a buffer of this size 3 times (reference kernel for HBM counters)")
@@ -236,6 +324,9 @@ def main():
         return
     if args.workload == "forced":
         forced_main(args)
+        return
+    if args.workload == "solver":
+        solver_main(args)
         return
     N = (args.N,) * 3
     if args.calibrate_copy:

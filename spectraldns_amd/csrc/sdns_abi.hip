@@ -504,7 +504,7 @@ template <typename Rt> int error_norm(sdns_plan* p, const void* err, const void*
                                       double rtol, double* out) {
   const int nblk = 1024;  // 3 * nblk partials fit the partials buffer (>= 8192 doubles)
   launch_errnorm<Rt>(err, u0, u1, (Rt)atol, (Rt)rtol, (long)p->N[0] * p->N[1] * p->Nh, p->part, nblk, p->stream);
-  if (launch_sum_rows(p->part, nblk, 3, out, p->stream)) return fail(SDNS_ERR_HIP, "sum rows");
+  if (launch_sum_cols(p->part, nblk, 3, out, p->stream)) return fail(SDNS_ERR_HIP, "sum columns");
   return 0;
 }
 
@@ -577,6 +577,26 @@ __global__ void __launch_bounds__(256) k_sum_rows(const double* __restrict__ par
   double s = 0;
   for (int b = 0; b < nblk; ++b) s += part[(long)b * width + j];
   out[j] = s;
+}
+// Column sums of part[nblk][width] with one block per column: strided per-thread
+// sums, then a fixed-order tree in LDS (deterministic, no serial chain over nblk).
+__global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ part, int nblk, int width,
+                                                  double* __restrict__ out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += 256) s += part[(long)b * width + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+int launch_sum_cols(const double* part, int nblk, int width, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_cols, dim3(width), dim3(256), 0, s, part, nblk, width, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_sum_rows(const double* part, int nblk, int width, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_rows, dim3((width + 255) / 256), dim3(256), 0, s, part, nblk, width, out);

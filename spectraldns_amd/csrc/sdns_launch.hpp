@@ -61,6 +61,8 @@ template <typename Rt> int launch_force_scale(const EwArgs<Rt>& a, Rt kf2, const
 // shells z = 1..nb (z = #{bins b_i = i + 1/2 : b_i < |K|}), out layout [nblk][2][nb].
 template <typename Rt> int launch_spectrum(const EwArgs<Rt>& a, int nb, double* partials, int nblk, hipStream_t s);
 int launch_sum_rows(const double* partials, int nblk, int width, double* out, hipStream_t s);
+// Same column sums, one block per column (for few columns over many rows).
+int launch_sum_cols(const double* partials, int nblk, int width, double* out, hipStream_t s);
 
 // ------------------------------------------------------------------ integrator updates
 // Linear combinations of state-shaped complex arrays, out = c[0] x[0] + c[1] x[1] + ...

@@ -110,9 +110,10 @@ def getConvection(convection):
         raise NotImplementedError("convection %r" % convection)
 
     def Conv(rhs, w_hat, work, Tp, VTp, K, K_over_K2, u_dealias=None):
-        u_hat = work[(w_hat, 1, True)]
+        # (both work arrays are written whole by the next launch: no zero fill)
+        u_hat = work[(w_hat, 1, False)]
         u_hat = cross2(u_hat, K_over_K2, w_hat)
-        v_hat = work[(rhs, 0, True)]
+        v_hat = work[(rhs, 0, False)]
         v_hat = VTp.plan.convection(u_hat, v_hat)  # F_k(u_dealias x w_dealias)
         rhs = cross2(rhs, K, v_hat)  # rhs = 1j*(K x v_hat)
         return rhs

@@ -104,6 +104,22 @@ int sdns_add_linear(sdns_plan* plan, void* rhs, const void* w_hat, double nu, co
 int sdns_dealias_truncate(sdns_plan* plan, void* out, const void* a);
 int sdns_elsasser_products(sdns_plan* plan, void* zz, const void* ub, size_t npoints);
 int sdns_set_elsasser(sdns_plan* plan, void* c_hat, const void* zz_hat);
+/* The Standard, Divergence and Skewed convection forms of the NS solver
+ * (spectralDNS/solvers/NS.py:138-199), 2/3 rule or no dealiasing:
+ *  gradient_truncated: out[j] = 1j*K_j*a on the kept modes, 0 elsewhere, for
+ *    one spectral component a (the input of Tp.backward(1j*K[j]*U_hat[i]) in
+ *    standard_convection, NS.py:138-145); SDNS_ERR_UNSUPPORTED for 3/2
+ *  dot3: out = u0*g0 + u1*g1 + u2*g2 on npoints physical points
+ *    (np.sum(u_dealias*gradUi, 0))
+ *  uu_products: uu = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) on npoints
+ *    physical points (the products of divergence_convection, NS.py:147-165)
+ *  div_uu: out = scale * 1j*sum_j K_j F(u_i u_j) from the six spectral
+ *    products, with the reference's grouping of the sums (scale -1 gives the
+ *    Divergence Conv, NS.py:180-185) */
+int sdns_gradient_truncated(sdns_plan* plan, void* out, const void* a_hat);
+int sdns_dot3(sdns_plan* plan, void* out, const void* u, const void* g, size_t npoints);
+int sdns_uu_products(sdns_plan* plan, void* uu, const void* u, size_t npoints);
+int sdns_div_uu(sdns_plan* plan, void* out, const void* uu_hat, double scale);
 int sdns_mask_nyquist(sdns_plan* plan, void* a_hat);
 
 /* Conv of getConvection('Vortex') (NS.py:191-198): rhs = VTp.forward(u x curl u)

@@ -37,6 +37,10 @@ _SIGS = {
     "sdns_dealias_truncate": (ctypes.c_int, [_P, _P, _P]),
     "sdns_elsasser_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_set_elsasser": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_gradient_truncated": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_dot3": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_size_t]),
+    "sdns_uu_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "sdns_div_uu": (ctypes.c_int, [_P, _P, _P, ctypes.c_double]),
     "sdns_mask_nyquist": (ctypes.c_int, [_P, _P]),
     "sdns_convection": (ctypes.c_int, [_P, _P, _P]),
     "sdns_compute_rhs": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
@@ -235,6 +239,51 @@ class Plan:
         self._sync_stream()
         _check(self._lib.sdns_elsasser_products(self._h, _ptr(zz), _ptr(ub), ub[0].numel()))
         return zz
+
+    def _physn(self, t, nc, name):
+        if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous() \
+                or tuple(t.shape) != ((nc,) if nc > 1 else ()) + self.physical_shape:
+            raise SdnsError("%s must be a contiguous %s %s GPU tensor of %d component(s)"
+                            % (name, self.physical_shape, self.real_dtype, nc))
+
+    def _specn(self, t, nc, name):
+        if t.device.type != "cuda" or t.dtype != self.complex_dtype or not t.is_contiguous() \
+                or tuple(t.shape) != ((nc,) if nc > 1 else ()) + self.spectral_shape:
+            raise SdnsError("%s must be a contiguous %s %s GPU tensor of %d component(s)"
+                            % (name, self.spectral_shape, self.complex_dtype, nc))
+
+    def gradient_truncated(self, out, a_hat):
+        """out[j] = 1j*K_j*a_hat on the kept modes, 0 elsewhere (one component in, three out)."""
+        self._specn(out, 3, "out")
+        self._specn(a_hat, 1, "a_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_gradient_truncated(self._h, _ptr(out), _ptr(a_hat)))
+        return out
+
+    def dot3(self, out, u, g):
+        """out = u0*g0 + u1*g1 + u2*g2 on physical arrays."""
+        self._physn(out, 1, "out")
+        self._physn(u, 3, "u")
+        self._physn(g, 3, "g")
+        self._sync_stream()
+        _check(self._lib.sdns_dot3(self._h, _ptr(out), _ptr(u), _ptr(g), out.numel()))
+        return out
+
+    def uu_products(self, uu, u):
+        """uu = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) on physical arrays."""
+        self._physn(uu, 6, "uu")
+        self._physn(u, 3, "u")
+        self._sync_stream()
+        _check(self._lib.sdns_uu_products(self._h, _ptr(uu), _ptr(u), u[0].numel()))
+        return uu
+
+    def div_uu(self, out, uu_hat, scale=1.0):
+        """out = scale * 1j*sum_j K_j F(u_i u_j) from the six spectral products."""
+        self._specn(out, 3, "out")
+        self._specn(uu_hat, 6, "uu_hat")
+        self._sync_stream()
+        _check(self._lib.sdns_div_uu(self._h, _ptr(out), _ptr(uu_hat), float(scale)))
+        return out
 
     def set_elsasser(self, c_hat, zz_hat):
         """c_hat (6 components) from the 9 spectral products zz_hat (MHD.py:104-113)."""

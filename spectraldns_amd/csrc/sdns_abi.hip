@@ -987,6 +987,23 @@ int sdns_set_elsasser(sdns_plan* p, void* c_hat, const void* zz_hat) {
   if (!zz_hat) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_SET_ELSASSER, c_hat, zz_hat, nullptr, nullptr, 0, -1);
 }
+int sdns_gradient_truncated(sdns_plan* p, void* out, const void* a) {
+  if (!a) return fail(SDNS_ERR_INVALID, "null input");
+  if (p && p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
+  return ew(p, EW_GRAD_TRUNC, out, a, nullptr, nullptr, 0, -1);
+}
+int sdns_dot3(sdns_plan* p, void* out, const void* u, const void* g, size_t npoints) {
+  if (!u || !g) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_DOT3, out, u, g, nullptr, 0, (long)npoints);
+}
+int sdns_uu_products(sdns_plan* p, void* uu, const void* u, size_t npoints) {
+  if (!u) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_UU_PRODUCTS, uu, u, nullptr, nullptr, 0, (long)npoints);
+}
+int sdns_div_uu(sdns_plan* p, void* out, const void* uu_hat, double scale) {
+  if (!uu_hat) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_DIV_UU, out, uu_hat, nullptr, nullptr, scale, -1);
+}
 int sdns_elsasser_products(sdns_plan* p, void* zz, const void* ub, size_t npoints) {
   if (!ub) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_ELSASSER_PRODUCTS, zz, ub, nullptr, nullptr, 0, (long)npoints);

@@ -199,6 +199,25 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) O[(3 * r + c) * n + i] = z0[r] * z1[c];
     return;
+  } else if constexpr (OP == EW_DOT3) {  // out = sum(u*g, 0) on n points (NS.py:138-145)
+    const Rt* U = (const Rt*)a.a;
+    const Rt* G = (const Rt*)a.b;
+    Rt s = U[i] * G[i];
+    s += U[n + i] * G[n + i];
+    s += U[2 * n + i] * G[2 * n + i];
+    ((Rt*)a.out)[i] = s;
+    return;
+  } else if constexpr (OP == EW_UU_PRODUCTS) {  // u_i u_j, i <= j, on n points (NS.py:147-165)
+    const Rt* U = (const Rt*)a.a;
+    Rt* O = (Rt*)a.out;
+    const Rt u0 = U[i], u1 = U[n + i], u2 = U[2 * n + i];
+    O[i] = u0 * u0;
+    O[n + i] = u0 * u1;
+    O[2 * n + i] = u0 * u2;
+    O[3 * n + i] = u1 * u1;
+    O[4 * n + i] = u1 * u2;
+    O[5 * n + i] = u2 * u2;
+    return;
   } else if constexpr (OP == EW_MUL_MINUS_I) {  // out = -1j * a (NS.py:96-99)
     const C v = ((const C*)a.a)[i];
     ((C*)a.out)[i] = mk<C>(v.y, -v.x);
@@ -248,6 +267,30 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
         }
         O[c * n + i] = r;
       }
+    } else if constexpr (OP == EW_GRAD_TRUNC) {  // out[j] = 1j*K_j*a on the kept modes (NS.py:141-142)
+      const int kx = ix < a.N0 / 2 ? ix : ix - a.N0, ky = iy < a.N1 / 2 ? iy : iy - a.N1;
+      const bool keep = (kx < 0 ? -kx : kx) <= a.keep[0] && (ky < 0 ? -ky : ky) <= a.keep[1] && kz <= a.keep[2];
+      const C v = A[i];
+      const Rt Kv[3] = {K0, K1, K2};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) O[j * n + i] = keep ? mk<C>(-Kv[j] * v.y, Kv[j] * v.x) : czero<C>();
+    } else if constexpr (OP == EW_DIV_UU) {  // divergence_convection (NS.py:147-165) times a.nu (+-1)
+      // a = F(u0u0, u0u1, u0u2, u1u1, u1u2, u2u2)
+      const C a00 = A[i], a01 = A[n + i], a02 = A[2 * n + i], a11 = A[3 * n + i], a12 = A[4 * n + i],
+              a22 = A[5 * n + i];
+      C s0 = mk<C>(K0 * a00.x, K0 * a00.y);
+      s0 = mk<C>(s0.x + K1 * a01.x, s0.y + K1 * a01.y);
+      s0 = mk<C>(s0.x + K2 * a02.x, s0.y + K2 * a02.y);
+      const C r0 = cmuli<1>(s0);
+      const C p1 = mk<C>(-K0 * a01.y, K0 * a01.x), q1 = mk<C>(-K1 * a11.y, K1 * a11.x),
+              t1 = mk<C>(-K2 * a12.y, K2 * a12.x);
+      const C r1 = mk<C>(p1.x + (q1.x + t1.x), p1.y + (q1.y + t1.y));
+      const C p2 = mk<C>(-K0 * a02.y, K0 * a02.x), q2 = mk<C>(-K1 * a12.y, K1 * a12.x),
+              t2 = mk<C>(-K2 * a22.y, K2 * a22.x);
+      const C r2 = mk<C>(p2.x + (q2.x + t2.x), p2.y + (q2.y + t2.y));
+      O[i] = mk<C>(a.nu * r0.x, a.nu * r0.y);
+      O[n + i] = mk<C>(a.nu * r1.x, a.nu * r1.y);
+      O[2 * n + i] = mk<C>(a.nu * r2.x, a.nu * r2.y);
     } else if constexpr (OP == EW_TRUNCATE) {  // 2/3-rule input truncation of a dealiased backward
       const int kx = ix < a.N0 / 2 ? ix : ix - a.N0, ky = iy < a.N1 / 2 ? iy : iy - a.N1;
       const bool keep = (kx < 0 ? -kx : kx) <= a.keep[0] && (ky < 0 ? -ky : ky) <= a.keep[1] && kz <= a.keep[2];
@@ -337,6 +380,10 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
     case EW_TRUNCATE: hipLaunchKernelGGL((k_elementwise<Rt, EW_TRUNCATE>), g, b, 0, s, a); return 0;
     case EW_SET_ELSASSER: hipLaunchKernelGGL((k_elementwise<Rt, EW_SET_ELSASSER>), g, b, 0, s, a); return 0;
     case EW_ELSASSER_PRODUCTS: hipLaunchKernelGGL((k_elementwise<Rt, EW_ELSASSER_PRODUCTS>), g, b, 0, s, a); return 0;
+    case EW_GRAD_TRUNC: hipLaunchKernelGGL((k_elementwise<Rt, EW_GRAD_TRUNC>), g, b, 0, s, a); return 0;
+    case EW_DOT3: hipLaunchKernelGGL((k_elementwise<Rt, EW_DOT3>), g, b, 0, s, a); return 0;
+    case EW_UU_PRODUCTS: hipLaunchKernelGGL((k_elementwise<Rt, EW_UU_PRODUCTS>), g, b, 0, s, a); return 0;
+    case EW_DIV_UU: hipLaunchKernelGGL((k_elementwise<Rt, EW_DIV_UU>), g, b, 0, s, a); return 0;
     default: return -1;
   }
 }

@@ -131,19 +131,87 @@ def Cross(c, a, b, work, T):
     return T.forward(d, c)
 
 
+def _dealiased_backward(plan, u_hat, u, tmp):
+    """VTp.backward(u_hat) for the 2/3 rule or no dealiasing: input truncation
+    to the kept modes, then the inverse transform."""
+    src = u_hat if plan.dealias == "None" else plan.dealias_truncate(tmp, u_hat)
+    return plan.backward(src, u)
+
+
+def standard_convection(rhs, u_dealias, U_hat, work, Tp, K):
+    """rhs_i = F(u_j du_i/dx_j) (NS.py:138-145): per component, the truncated
+    gradient 1j*K_j*U_hat_i, its inverse transform, the dot product with
+    u_dealias and the forward transform."""
+    plan = K.plan
+    gradUi = work[(u_dealias, 1, False)]
+    g_hat = work[(U_hat, 2, False)]
+    s = work[(u_dealias[0], 0, False)]
+    for i in range(3):
+        plan.gradient_truncated(g_hat, U_hat[i])
+        plan.backward(g_hat, gradUi)
+        plan.dot3(s, u_dealias, gradUi)
+        plan.forward(s, rhs[i])
+    return rhs
+
+
+def divergence_convection(rhs, u_dealias, work, Tp, K, add=False, scale=1.0):
+    """rhs_i (+)= scale * 1j*sum_j K_j F(u_i u_j) (NS.py:147-165): one kernel for
+    the six products, their forward transforms, one kernel for the sums."""
+    import torch
+    plan = K.plan
+    bufs = getattr(plan, "_uu_buffers", None)
+    if bufs is None:
+        bufs = plan._uu_buffers = (
+            torch.empty((6,) + tuple(plan.physical_shape), dtype=plan.real_dtype, device=plan.device),
+            torch.empty((6,) + tuple(plan.spectral_shape), dtype=plan.complex_dtype, device=plan.device))
+    uu, uu_hat = bufs
+    plan.uu_products(uu, u_dealias)
+    plan.forward(uu[0:3], uu_hat[0:3])
+    plan.forward(uu[3:6], uu_hat[3:6])
+    if add:
+        d = work[(rhs, 3, False)]
+        plan.div_uu(d, uu_hat, scale)
+        plan.lincomb(rhs, [rhs, d], [1.0, 1.0])
+    else:
+        plan.div_uu(rhs, uu_hat, scale)
+    return rhs
+
+
 def getConvection(convection):
-    """Return Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias) (NS.py:164-201).
+    """Return Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias) (NS.py:167-201).
 
     The Vortex form F(u x curl u) on the dealiased space is one fused launch
-    sequence; the Standard, Divergence and Skewed forms are not built."""
-    if convection != "Vortex":
-        raise NotImplementedError("convection %r is not built; the HIP path implements Vortex" % convection)
+    sequence (2/3 and 3/2 rule).  The Standard, Divergence and Skewed forms
+    run their transforms and products as separate launches, for the 2/3 rule
+    and without dealiasing (the 3/2 rule raises: its padded transforms exist
+    only inside the fused passes)."""
+    if convection == "Vortex":
+        def Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias=None):
+            return VTp.plan.convection(u_hat, rhs)
 
-    def Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias=None):
-        return VTp.plan.convection(u_hat, rhs)
+        Conv.convection = convection
+        Conv.fused_vortex = True
+        return Conv
+    if convection not in ("Standard", "Divergence", "Skewed"):
+        raise NotImplementedError("convection %r" % convection)
+
+    def Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias):
+        plan = VTp.plan
+        if plan.dealias == "3/2-rule":
+            raise NotImplementedError("the %s form is built for the 2/3 rule and no dealiasing" % convection)
+        u_dealias = _dealiased_backward(plan, u_hat, u_dealias, work[(u_hat, 4, False)])
+        if convection == "Standard":
+            rhs = standard_convection(rhs, u_dealias, u_hat, work, Tp, K)
+            plan.lincomb(rhs, [rhs], [-1.0])
+        elif convection == "Divergence":
+            rhs = divergence_convection(rhs, u_dealias, work, Tp, K, False, scale=-1.0)
+        else:
+            rhs = standard_convection(rhs, u_dealias, u_hat, work, Tp, K)
+            rhs = divergence_convection(rhs, u_dealias, work, Tp, K, True)
+            plan.lincomb(rhs, [rhs], [-0.5])
+        return rhs
 
     Conv.convection = convection
-    Conv.fused_vortex = True
     return Conv
 
 
@@ -164,7 +232,13 @@ def ComputeRHS(rhs, u_hat, solver, **context):
     src = Source if Source._version > 0 else None
     if getattr(solver.conv, "fused_vortex", False):
         return VT.plan.compute_rhs(u_hat, rhs, params.nu, source=src, p_hat=c["P_hat"])
-    rhs = solver.conv(rhs, u_hat, c["work"], c["Tp"], c["VTp"], c["K"], None)
+    u_dealias = c.get("u_dealias")
+    if u_dealias is None:  # (the context builds it on first access; keyword unpacking skips it)
+        VTp = c["VTp"]
+        u_dealias = getattr(VTp, "_u_dealias", None)
+        if u_dealias is None:
+            u_dealias = VTp._u_dealias = Array(VTp)
+    rhs = solver.conv(rhs, u_hat, c["work"], c["Tp"], c["VTp"], c["K"], u_dealias)
     if c["mask"] is not None:
         VT.plan.mask_nyquist(rhs)
     rhs = solver.add_pressure_diffusion(rhs, u_hat, params.nu, None, c["K"], c["P_hat"], None)

@@ -1,0 +1,77 @@
+"""The Standard, Divergence and Skewed convection forms of the NS solver
+(spectralDNS/solvers/NS.py:138-199) on the GPU, through the reference's API
+(config convection, get_solver, solve; Taylor-Green pins at ntol 7 as in
+tests/TG.py:116-126) on the uniform and the non-uniform box, and against the
+oracle's restatement (oracle/convection.py) on the same inputs: FP64 within
+1e-12 relative L2 after the ten RK4 steps.  The 3/2 rule raises for these
+forms."""
+import numpy as np
+import pytest
+
+from oracle import convection as OCV
+from oracle.ns import Grid, dt_schedule
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+TWO_PI = 2 * np.pi
+CASES = [["--M", "4", "4", "4", "--L", "2*pi", "2*pi", "2*pi"],
+         ["--M", "6", "5", "4", "--L", "6*pi", "4*pi", "2*pi"]]
+
+
+def tg_initialize(solver, context):
+    from spectraldns_amd import config
+    U, X = context.U, context.X
+    U[0] = torch.sin(X[0]) * torch.cos(X[1]) * torch.cos(X[2])
+    U[1] = -torch.cos(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
+    U[2] = 0
+    solver.set_velocity(**context)
+    config.params.t = 0.0
+    config.params.tstep = 0
+
+
+def regression(context):
+    from spectraldns_amd import config
+    solver = config.solver
+    U = solver.get_velocity(**context)
+    curl = solver.get_curl(**context)
+    n = np.prod(config.params.N)
+    w = float(torch.sum(curl.double() * curl.double()).item()) / n / 2
+    k = float(torch.sum(U.double() * U.double()).item()) / n / 2
+    assert round(w - 0.375249930801, 7) == 0, w
+    assert round(k - 0.124953117517, 7) == 0, k
+
+
+@pytest.mark.parametrize("form", ["Standard", "Divergence", "Skewed"])
+@pytest.mark.parametrize("args", CASES)
+def test_convection_forms_fp64(args, form):
+    from spectraldns_amd import config, get_solver, solve
+    print("\n[case] NS %s %s" % (form, " ".join(args)), flush=True)
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "convection": form, "dealias": "2/3-rule",
+                   "precision": "double"})
+    solver = get_solver(regression_test=regression, parse_args=args + ["NS"])
+    c = solver.get_context()
+    tg_initialize(solver, c)
+    solve(solver, c)
+    u = c.U_hat.cpu().numpy()
+    config.update({"convection": "Vortex"})
+    M = [int(a) for a in args[1:4]]
+    L = (TWO_PI * 3, TWO_PI * 2, TWO_PI) if M[0] == 6 else (TWO_PI,) * 3
+    g = Grid(tuple(2 ** m for m in M), L, "double", "2/3-rule")
+    ref = g.taylor_green()
+    for dt in dt_schedule(0.1, 0.01):
+        ref = OCV.rk4_step(g, ref, dt, 0.000625, form)
+    err = float(np.linalg.norm((u - ref).ravel()) / np.linalg.norm(ref.ravel()))
+    assert err < 1e-12, err
+
+
+def test_convection_forms_reject_32_rule():
+    from spectraldns_amd import config, get_solver
+    config.update({"nu": 0.01, "dt": 0.01, "T": 0.02, "convection": "Divergence", "dealias": "3/2-rule",
+                   "precision": "double"})
+    solver = get_solver(parse_args=["--M", "4", "4", "4", "NS"])
+    c = solver.get_context()
+    solver.conv = solver.getConvection("Divergence")
+    with pytest.raises(NotImplementedError):
+        solver.ComputeRHS(c.dU, c.U_hat, solver, **c)
+    config.update({"convection": "Vortex", "dealias": "2/3-rule"})

@@ -33,6 +33,22 @@ def initialize(solver, context):
     config.params.tstep = 0
 
 
+def init_from_file(filename, solver, context):
+    """Restart from a checkpoint (Isotropic.py:267-282): U_hat from its
+    ``U/3D/0`` data, the mean mode zeroed, W_hat for the VV solver, and the
+    target energy of the forcing.  Also returns the checkpoint's t and tstep."""
+    from spectraldns_amd.h5io import read_checkpoint
+    f = read_checkpoint(filename)
+    assert "U/3D/0" in f
+    U_hat = context.U_hat
+    U_hat.copy_(torch.from_numpy(f["U/3D/0"]).to(U_hat.dtype))
+    U_hat[:, 0, 0, 0] = 0.0
+    if "VV" in config.params.solver:
+        context.W_hat = solver.cross2(context.W_hat, context.K, context.U_hat)
+    context.target_energy = energy_fourier(U_hat, context.T)
+    return f["t"], f["tstep"]
+
+
 def update(context):
     c = context
     params = config.params
@@ -59,8 +75,11 @@ if __name__ == "__main__":
     config.triplyperiodic.add_argument("--k0", type=float, default=4.0)
     config.triplyperiodic.add_argument("--energy", type=float, default=0.5)
     config.triplyperiodic.add_argument("--seed", type=int, default=1234)
+    config.triplyperiodic.add_argument("--restart", default="", help="checkpoint (<name>_c.npz) to start from")
     sol = get_solver(update=update, mesh="triplyperiodic")
     context = sol.get_context()
     context.spectra = {}
     initialize(sol, context)
+    if config.params.restart:
+        config.params.t, config.params.tstep = init_from_file(config.params.restart, sol, context)
     solve(sol, context)

@@ -12,7 +12,20 @@ import sys
 
 import numpy as np
 
-__all__ = ["HDF5File"]
+__all__ = ["HDF5File", "read_checkpoint"]
+
+
+def read_checkpoint(filename):
+    """Arrays, t and tstep of a checkpoint written by HDF5File.update: the file
+    ``<filename>_c.npz`` (or ``filename`` itself when it ends in .npz) as a
+    dict keyed like the reference's HDF5 datasets (``U/3D/0``), with ``t``
+    (float) and ``tstep`` (int).  Loads nothing executable (no pickles)."""
+    fn = filename if filename.endswith(".npz") else filename + "_c.npz"
+    with np.load(fn, allow_pickle=False) as f:
+        out = {k: f[k] for k in f.files}
+    out["t"] = float(out["t"])
+    out["tstep"] = int(out["tstep"])
+    return out
 
 
 def _host(a):

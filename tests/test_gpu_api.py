@@ -107,3 +107,39 @@ def test_compute_rhs_granular_path():
     g = Grid((16, 16, 16))
     ref = g.compute_rhs(u, 0.01)
     assert np.max(abs(rhs.cpu().numpy() - ref)) < 1e-13 * np.max(abs(ref))
+
+
+def test_checkpoint_restart_continues_bitwise(tmp_path, monkeypatch):
+    """A run checkpointing every 5 steps (HDF5File.update, HDF5File.py:57-120,
+    with the reference's keys U/3D/0, t, tstep), restarted from its step-5
+    checkpoint, reaches t = 0.1 with the same U_hat, bit for bit, as the run
+    itself."""
+    import shutil
+    from spectraldns_amd import config, get_solver, solve
+    from spectraldns_amd.h5io import read_checkpoint
+    monkeypatch.chdir(tmp_path)
+
+    def keep_step5(context):  # update runs before the checkpoint of its step
+        if config.params.tstep == 6:
+            shutil.copy("NS_c.npz", "step5_c.npz")
+
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "dealias": "2/3-rule", "precision": "double",
+                   "checkpoint": 5, "write_result": 100000})
+    solver = get_solver(update=keep_step5, regression_test=lambda c: None,
+                        parse_args=["--M", "4", "4", "4", "NS"])
+    c = solver.get_context()
+    tg_initialize(solver, c)
+    solve(solver, c)
+    straight = c.U_hat.clone()
+    f10 = read_checkpoint("NS")
+    assert f10["tstep"] == 10 and abs(f10["t"] - 0.1) < 1e-12
+    assert np.array_equal(f10["U/3D/0"], straight.cpu().numpy())
+    f = read_checkpoint("step5")
+    assert f["tstep"] == 5
+    solver.update = lambda c: None
+    c2 = solver.get_context()
+    c2.U_hat.copy_(torch.from_numpy(f["U/3D/0"]))
+    config.params.t, config.params.tstep = f["t"], f["tstep"]
+    solve(solver, c2)
+    assert torch.equal(c2.U_hat, straight)
+    config.update({"checkpoint": 100000, "write_result": 100000000})

@@ -75,3 +75,25 @@ def test_convection_forms_reject_32_rule():
     with pytest.raises(NotImplementedError):
         solver.ComputeRHS(c.dU, c.U_hat, solver, **c)
     config.update({"convection": "Vortex", "dealias": "2/3-rule"})
+
+
+@pytest.mark.parametrize("form", ["Standard", "Skewed"])
+def test_convection_forms_fp32(form):
+    """Single precision against the FP64 oracle on the 32^3 box (1e-5)."""
+    from spectraldns_amd import config, get_solver, solve
+    print("\n[case] NS %s 32^3 single" % form, flush=True)
+    config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "convection": form, "dealias": "2/3-rule",
+                   "precision": "single"})
+    solver = get_solver(update=lambda c: None, regression_test=lambda c: None,
+                        parse_args=["--M", "5", "5", "5", "NS"])
+    c = solver.get_context()
+    tg_initialize(solver, c)
+    solve(solver, c)
+    u = c.U_hat.cpu().numpy().astype(np.complex128)
+    config.update({"convection": "Vortex", "precision": "double"})
+    g = Grid((32, 32, 32), (TWO_PI,) * 3, "double", "2/3-rule")
+    ref = g.taylor_green()
+    for dt in dt_schedule(0.1, 0.01, precision="single"):
+        ref = OCV.rk4_step(g, ref, dt, 0.000625, form)
+    err = float(np.linalg.norm((u - ref).ravel()) / np.linalg.norm(ref.ravel()))
+    assert err < 1e-5, err

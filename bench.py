@@ -214,8 +214,9 @@ def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
 # Algorithmic HBM bytes per time step of the solver workloads, in units of
 # F = N0*N1*(N2/2+1)*esz (one complex spectral field; a real physical field is
 # counted as F too), 2/3 rule.  VV RHS: K/K^2 curl-inverse 6F, fused
-# convection passes 40F (P1-P4 34F + convection-only x-forward 6F), cross2 6F,
-# add_linear 9F = 61F; RK4 stage by stage: two state copies 12F, three
+# convection passes 40F (P1-P4 34F + convection-only x-forward 6F), the
+# one-launch tail (cross2, mask, add_linear: reads F(u x w) and w_hat, writes
+# rhs) 9F = 55F; RK4 stage by stage: two state copies 12F, three
 # u0 = u1 + c rhs 27F, four u2 += c rhs 36F, final copy 6F = 81F.  MHD RHS:
 # truncation 12F, six inverse transforms 36F (x, y, z pass: 6F per field),
 # Elsasser products 15F, nine forward transforms 54F, set_Elsasser 15F,
@@ -226,7 +227,7 @@ def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
 # after the first) stage sums sum_{i=1..7} 3(i+2)F = 126F, solution sum 30F,
 # error sum 27F, error norm 9F, copy of the new state 6F.
 SOLVER_BYTES_F = {
-    ("VV", "RK4"): 4 * 61 + 81,
+    ("VV", "RK4"): 4 * 55 + 81,
     ("MHD", "RK4"): 4 * 151 + 162,
     ("NS", "ForwardEuler"): 44 + 9,
     ("NS", "AB2"): 44 + 18,

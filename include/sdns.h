@@ -89,6 +89,12 @@ int sdns_mul_minus_i(sdns_plan* plan, void* out, const void* a, size_t n);
  *    (add_linear, VV.py:108-113), in place on rhs */
 int sdns_cross2_k_over_k2(sdns_plan* plan, void* u_hat, const void* w_hat);
 int sdns_add_linear(sdns_plan* plan, void* rhs, const void* w_hat, double nu, const void* source);
+/* The VV right-hand side after the convection passes, in one launch:
+ * rhs = 1j*(K x v_hat) (Conv, VV.py:104-106), Nyquist planes zeroed when the
+ * plan masks them (VV.py:143-144), then rhs -= nu*K2*w_hat and rhs += source
+ * when not NULL (add_linear, VV.py:110-115); v_hat = F(u x w). */
+int sdns_vv_rhs_tail(sdns_plan* plan, void* rhs, const void* v_hat, const void* w_hat, double nu,
+                     const void* source);
 /* MHD solver in Elsasser form (spectralDNS/solvers/MHD.py), 2/3 rule or no
  * dealiasing:
  *  dealias_truncate: out = a with the modes outside the plan's kept input

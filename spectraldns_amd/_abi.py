@@ -34,6 +34,7 @@ _SIGS = {
     "sdns_mul_minus_i": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_cross2_k_over_k2": (ctypes.c_int, [_P, _P, _P]),
     "sdns_add_linear": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
+    "sdns_vv_rhs_tail": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, _P]),
     "sdns_dealias_truncate": (ctypes.c_int, [_P, _P, _P]),
     "sdns_elsasser_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_set_elsasser": (ctypes.c_int, [_P, _P, _P]),
@@ -295,6 +296,18 @@ class Plan:
         self._sync_stream()
         _check(self._lib.sdns_set_elsasser(self._h, _ptr(c_hat), _ptr(zz_hat)))
         return c_hat
+
+    def vv_rhs_tail(self, rhs, v_hat, w_hat, nu, source=None):
+        """rhs = mask(1j*(K x v_hat)) - nu*K2*w_hat (+ source): the VV RHS after
+        the convection passes, one launch."""
+        for t, nm in ((rhs, "rhs"), (v_hat, "v_hat"), (w_hat, "w_hat")):
+            self._spec(t, 3, nm)
+        if source is not None:
+            self._spec(source, 3, "Source")
+        self._sync_stream()
+        _check(self._lib.sdns_vv_rhs_tail(self._h, _ptr(rhs), _ptr(v_hat), _ptr(w_hat), float(nu),
+                                          _ptr(source) if source is not None else None))
+        return rhs
 
     def add_linear(self, rhs, w_hat, nu, source=None):
         """rhs -= nu*K2*w_hat; rhs += source (solvers/VV.py:110-115), in place."""

@@ -978,6 +978,10 @@ int sdns_add_linear(sdns_plan* p, void* rhs, const void* w_hat, double nu, const
   if (!w_hat) return fail(SDNS_ERR_INVALID, "null input");
   return ew(p, EW_ADD_LINEAR, rhs, w_hat, source, nullptr, nu, -1);
 }
+int sdns_vv_rhs_tail(sdns_plan* p, void* rhs, const void* v_hat, const void* w_hat, double nu, const void* source) {
+  if (!v_hat || !w_hat) return fail(SDNS_ERR_INVALID, "null input");
+  return ew(p, EW_VV_TAIL, rhs, v_hat, w_hat, const_cast<void*>(source), nu, -1);
+}
 int sdns_dealias_truncate(sdns_plan* p, void* out, const void* a) {
   if (!a) return fail(SDNS_ERR_INVALID, "null input");
   if (p && p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");

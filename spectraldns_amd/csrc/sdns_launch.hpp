@@ -47,7 +47,7 @@ template <typename Rt> struct EwArgs {
 enum EwOp { EW_CROSS1 = 0, EW_CROSS2 = 1, EW_PRESSURE_DIFFUSION = 2, EW_DIVERGENCE = 3,
             EW_MUL_MINUS_I = 4, EW_MASK_NYQUIST = 5, EW_SGS_FORCE = 6, EW_CROSS2_KOK = 7, EW_ADD_LINEAR = 8,
             EW_TRUNCATE = 9, EW_SET_ELSASSER = 10, EW_ELSASSER_PRODUCTS = 11, EW_GRAD_TRUNC = 12, EW_DOT3 = 13,
-            EW_UU_PRODUCTS = 14, EW_DIV_UU = 15 };
+            EW_UU_PRODUCTS = 14, EW_DIV_UU = 15, EW_VV_TAIL = 16 };
 template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipStream_t s);
 // FP64 (energy_fourier(u_hat), energy_fourier(curl_hat)) partial sums
 template <typename Rt> int launch_energy(const EwArgs<Rt>& a, double* partials, int nblk, hipStream_t s);

@@ -251,6 +251,30 @@ __global__ void __launch_bounds__(256) k_elementwise(EwArgs<Rt> a) {
       O[i] = cmuli<1>(c0);
       O[n + i] = cmuli<1>(c1);
       O[2 * n + i] = cmuli<1>(c2);
+    } else if constexpr (OP == EW_VV_TAIL) {  // VV.py:104-146 after F(u x w): cross2, mask, add_linear
+      // a = v_hat = F(u x w), b = w_hat, aux = Source (may be null); out = rhs
+      const C v0 = A[i], v1 = A[n + i], v2 = A[2 * n + i];
+      const C c0 = cmuli<1>(mk<C>(K1 * v2.x - K2 * v1.x, K1 * v2.y - K2 * v1.y));
+      const C c1 = cmuli<1>(mk<C>(K2 * v0.x - K0 * v2.x, K2 * v0.y - K0 * v2.y));
+      const C c2 = cmuli<1>(mk<C>(K0 * v1.x - K1 * v0.x, K0 * v1.y - K1 * v0.y));
+      const bool nyq = a.mask_nyquist && (ix == a.N0 / 2 || iy == a.N1 / 2 || kz == a.N2 / 2);
+      Rt k2 = K0 * K0;
+      k2 += K1 * K1;
+      k2 += K2 * K2;
+      const Rt z = a.nu * k2;
+      const C* W = (const C*)a.b;
+      const C cc[3] = {c0, c1, c2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const C w = W[c * n + i];
+        C r = nyq ? czero<C>() : cc[c];
+        r = mk<C>(r.x - z * w.x, r.y - z * w.y);
+        if (a.aux) {
+          const C src = ((const C*)a.aux)[c * n + i];
+          r = mk<C>(r.x + src.x, r.y + src.y);
+        }
+        O[c * n + i] = r;
+      }
     } else if constexpr (OP == EW_ADD_LINEAR) {  // rhs -= nu*K2*w_hat; rhs += Source (VV.py:108-113)
       Rt k2 = K0 * K0;
       k2 += K1 * K1;
@@ -384,6 +408,7 @@ template <typename Rt> int launch_elementwise(int op, const EwArgs<Rt>& a, hipSt
     case EW_DOT3: hipLaunchKernelGGL((k_elementwise<Rt, EW_DOT3>), g, b, 0, s, a); return 0;
     case EW_UU_PRODUCTS: hipLaunchKernelGGL((k_elementwise<Rt, EW_UU_PRODUCTS>), g, b, 0, s, a); return 0;
     case EW_DIV_UU: hipLaunchKernelGGL((k_elementwise<Rt, EW_DIV_UU>), g, b, 0, s, a); return 0;
+    case EW_VV_TAIL: hipLaunchKernelGGL((k_elementwise<Rt, EW_VV_TAIL>), g, b, 0, s, a); return 0;
     default: return -1;
   }
 }

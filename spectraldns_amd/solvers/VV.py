@@ -110,16 +110,23 @@ def getConvection(convection):
         raise NotImplementedError("convection %r" % convection)
 
     def Conv(rhs, w_hat, work, Tp, VTp, K, K_over_K2, u_dealias=None):
-        # (both work arrays are written whole by the next launch: no zero fill)
-        u_hat = work[(w_hat, 1, False)]
-        u_hat = cross2(u_hat, K_over_K2, w_hat)
-        v_hat = work[(rhs, 0, False)]
-        v_hat = VTp.plan.convection(u_hat, v_hat)  # F_k(u_dealias x w_dealias)
+        v_hat = _cross_convection(w_hat, work, VTp, K_over_K2)
         rhs = cross2(rhs, K, v_hat)  # rhs = 1j*(K x v_hat)
         return rhs
 
     Conv.convection = convection
+    Conv.vv_fused_tail = True
     return Conv
+
+
+def _cross_convection(w_hat, work, VTp, K_over_K2):
+    """v_hat = F_k(u_dealias x w_dealias) with u_hat = 1j*(K/K^2) x w_hat (the
+    dealiased convection passes).  Both work arrays are written whole by the
+    next launch, so they are not zero-filled."""
+    u_hat = work[(w_hat, 1, False)]
+    u_hat = cross2(u_hat, K_over_K2, w_hat)
+    v_hat = work[(w_hat, 0, False)]
+    return VTp.plan.convection(u_hat, v_hat)
 
 
 @optimizer
@@ -134,6 +141,10 @@ def ComputeRHS(rhs, w_hat, solver, **context):
     c = context
     Source = c["Source"]
     src = Source if Source._version > 0 else None
+    if getattr(solver.conv, "vv_fused_tail", False) and solver.add_linear is add_linear:
+        # the module's own Conv and add_linear: cross2, mask and add_linear in one launch
+        v_hat = _cross_convection(w_hat, c["work"], c["VTp"], c["K_over_K2"])
+        return c["VT"].plan.vv_rhs_tail(rhs, v_hat, w_hat, params.nu, src)
     rhs = solver.conv(rhs, w_hat, c["work"], c["Tp"], c["VTp"], c["K"], c["K_over_K2"], None)
     if c["mask"] is not None:
         c["VT"].plan.mask_nyquist(rhs)

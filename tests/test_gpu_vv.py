@@ -111,3 +111,29 @@ def test_vv_kernels(precision):
     src = torch.from_numpy(0.5 * w).to(cd).cuda()
     p.add_linear(rhs, wt, 0.02, src)
     assert rel(rhs.cpu().numpy(), u - 0.02 * g.K2 * w + 0.5 * w) < 10 * tol
+
+
+def test_vv_fused_tail_equals_granular_path():
+    """The one-launch VV RHS tail (cross2, Nyquist mask, add_linear with a
+    Source) equals the step-by-step kernels of VV.py:117-146 bit for bit."""
+    from spectraldns_amd import config, get_solver
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    config.update({"nu": 0.01, "dt": 0.01, "T": 0.02, "dealias": "2/3-rule", "precision": "double"})
+    solver = get_solver(update=lambda c: None, regression_test=lambda c: None,
+                        parse_args=["--M", "4", "4", "4", "VV"])
+    c = solver.get_context()
+    g = Grid((16, 16, 16), (TWO_PI,) * 3, "double", "2/3-rule")
+    u = random_solenoidal_field((16, 16, 16), seed=12, k0=3.0, energy=0.5)
+    c.W_hat.copy_(torch.from_numpy(g.cross2(u)))
+    c.Source.copy_(torch.from_numpy(0.01 * g.cross2(u)))
+    solver.conv = solver.getConvection("Vortex")
+    fused = solver.ComputeRHS(c.dU, c.W_hat, solver, **c).clone()
+    base = solver.add_linear
+    solver.add_linear = lambda *a: base(*a)  # a replaced add_linear takes the step-by-step path
+    try:
+        granular = solver.ComputeRHS(c.dU, c.W_hat, solver, **c).clone()
+    finally:
+        solver.add_linear = base
+    assert torch.equal(fused, granular)
+    ref = OV.compute_rhs(g, g.cross2(u), 0.01, source=0.01 * g.cross2(u))
+    assert rel(fused.cpu().numpy(), ref) < 1e-13

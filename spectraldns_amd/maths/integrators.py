@@ -144,6 +144,8 @@ def adaptiveRK(A, b, bhat, err_order, fY_hat, u0_new, sc, err, fsal, offset, aTO
 def getintegrator(rhs, u0, solver, context):
     """Return the integrator chosen by params.integrator (integrators.py:177-239)."""
     params = solver.params
+    if params.integrator not in ("RK4", "ForwardEuler", "AB2", "BS5_adaptive", "BS5_fixed"):
+        raise NotImplementedError("integrator %r" % params.integrator)
     if params.integrator == "RK4" and not (getattr(solver.conv, "fused_vortex", False) and
                                            solver.ComputeRHS is getattr(solver, "_ComputeRHS_fused", None)):
         if params.get("compensated", False):

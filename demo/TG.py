@@ -5,6 +5,10 @@ regression test k(0.1) = 0.124953117517, w(0.1) = 0.375249930801 (tests/TG.py:11
 
     python demo/TG.py --M 5 5 5 NS
     python demo/TG.py --dealias 3/2-rule --precision single --ntol 5 NS
+    python demo/TG.py --integrator BS5_adaptive VV
+
+For the VV solver the state is the vorticity W_hat = 1j*(K x U_hat)
+(tests/TG.py:29-35).
 """
 import os
 import sys
@@ -23,6 +27,8 @@ def initialize(solver, context):
     U[1] = -torch.cos(X[0]) * torch.sin(X[1]) * torch.cos(X[2])
     U[2] = 0
     solver.set_velocity(**context)
+    if "VV" in config.params.solver:
+        solver.cross2(context.W_hat, context.K, context.U_hat)
     config.params.t = 0.0
     config.params.tstep = 0
 
@@ -37,6 +43,8 @@ def update(context):
     if params.tstep % params.compute_energy == 0:
         U = solver.get_velocity(**context)
         curl = solver.get_curl(**context)
+        if "VV" in params.solver:  # U_hat from the velocity the vorticity gives
+            solver.set_velocity(**context)
         kk = _mean_sq(U)
         ww = _mean_sq(curl)
         ww2 = energy_fourier(context.U_hat, context.T) / 2

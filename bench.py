@@ -218,9 +218,10 @@ def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
 # one-launch tail (cross2, mask, add_linear: reads F(u x w) and w_hat, writes
 # rhs) 9F = 55F; RK4 stage by stage: three stage values u0 + c rhs 27F, the
 # accumulator u2 = u0 + c rhs then three u2 (+)= c rhs 36F = 63F.  MHD RHS:
-# truncation 12F, six inverse transforms 36F (x, y, z pass: 6F per field),
-# Elsasser products 15F, nine forward transforms 54F, set_Elsasser 15F,
-# pressure/diffusion of U 10F, diffusion of B 9F = 151F; RK4 over six
+# six inverse transforms of the kept modes 36F (x, y, z pass: 6F per field,
+# the 2/3-rule truncation inside them), Elsasser products 15F, nine forward
+# transforms 54F, set_Elsasser 15F, pressure/diffusion of U 10F, diffusion
+# of B 9F = 139F; RK4 over six
 # components 126F.  NS integrators: fused RHS 44F (P1-P4 34F; x-forward with
 # the projection reads W4 and u_hat, writes rhs and P_hat: 10F);
 # ForwardEuler update 9F; AB2 updates 12F + 6F; BS5 (FSAL: 7 RHS per step
@@ -228,7 +229,7 @@ def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
 # error sum 27F, error norm 9F, copy of the new state 6F.
 SOLVER_BYTES_F = {
     ("VV", "RK4"): 4 * 55 + 63,
-    ("MHD", "RK4"): 4 * 151 + 126,
+    ("MHD", "RK4"): 4 * 139 + 126,
     ("NS", "ForwardEuler"): 44 + 9,
     ("NS", "AB2"): 44 + 18,
     ("NS", "BS5_fixed"): 7 * 44 + 126 + 30 + 27 + 9 + 6,

@@ -108,6 +108,11 @@ int sdns_vv_rhs_tail(sdns_plan* plan, void* rhs, const void* v_hat, const void* 
  *    c_hat[3:6] = 1j*sum_j K_j (zz_ji - zz_ij)/2 from the 9 spectral
  *    products (set_Elsasser, MHD.py:104-113) */
 int sdns_dealias_truncate(sdns_plan* plan, void* out, const void* a);
+/* dealias_truncate followed by sdns_backward, in the transform's own passes
+ * (each pass reads only the kept modes of its axis): VMp.backward / VTp.backward
+ * of shenfun's 2/3 rule (dealias_direct) or of no dealiasing; ncomp <= 3;
+ * SDNS_ERR_UNSUPPORTED for a 3/2-rule plan. */
+int sdns_backward_dealiased(sdns_plan* plan, const void* u_hat, void* u, int ncomp);
 int sdns_elsasser_products(sdns_plan* plan, void* zz, const void* ub, size_t npoints);
 int sdns_set_elsasser(sdns_plan* plan, void* c_hat, const void* zz_hat);
 /* The Standard, Divergence and Skewed convection forms of the NS solver

@@ -36,6 +36,7 @@ _SIGS = {
     "sdns_add_linear": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P]),
     "sdns_vv_rhs_tail": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, _P]),
     "sdns_dealias_truncate": (ctypes.c_int, [_P, _P, _P]),
+    "sdns_backward_dealiased": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
     "sdns_elsasser_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_set_elsasser": (ctypes.c_int, [_P, _P, _P]),
     "sdns_gradient_truncated": (ctypes.c_int, [_P, _P, _P]),
@@ -195,6 +196,16 @@ class Plan:
             raise SdnsError("component mismatch")
         self._sync_stream()
         _check(self._lib.sdns_backward(self._h, _ptr(u_hat), _ptr(u), nc))
+        return u
+
+    def backward_dealiased(self, u_hat, u):
+        """Inverse transform of the kept input modes only (the 2/3-rule or
+        undealiased VTp.backward), the truncation done inside the passes."""
+        nc = self._spec(u_hat, None, "u_hat")
+        if self._phys(u, "u") != nc:
+            raise SdnsError("component mismatch")
+        self._sync_stream()
+        _check(self._lib.sdns_backward_dealiased(self._h, _ptr(u_hat), _ptr(u), nc))
         return u
 
     def cross1(self, c, a, b):

@@ -265,7 +265,10 @@ void tstop(sdns_plan* p) {
 }
 
 // ---------------------------------------------------------------- transforms
-template <typename Rt> int plain_backward(sdns_plan* p, const void* uh, void* up, int nc) {
+// With trunc, each pass reads only the plan's kept input modes of its axis (the
+// others count as zero): the tensor-product truncation of a 2/3-rule
+// dealiased backward transform, inside the transform.
+template <typename Rt> int plain_backward(sdns_plan* p, const void* uh, void* up, int nc, bool trunc = false) {
   const int N0 = p->N[0], N1 = p->N[1], Nh = p->Nh, N2 = p->N[2];
   const long plane = (long)N1 * Nh, fs = (long)N0 * plane;
   StridedArgs<Rt> a{};
@@ -273,17 +276,20 @@ template <typename Rt> int plain_backward(sdns_plan* p, const void* uh, void* up
   a.in = uh; a.out = p->bufA; a.tw = p->tw[N0];
   a.Q = (int)plane; a.cdiv = (int)plane; a.cvalid = (int)plane; a.in_hi = 0; a.out_hi = 0;
   a.in_row = plane; a.out_row = plane; a.in_field = fs; a.out_field = fs;
-  a.rin = rmap(N0 / 2, N0 / 2 - 1, N0); a.rout = a.rin; a.scale = Rt(1);
+  a.rout = rmap(N0 / 2, N0 / 2 - 1, N0);
+  a.rin = trunc ? rmap(p->in[0][0], p->in[0][1], N0) : a.rout;
+  a.scale = Rt(1);
   if (launch_strided<Rt>(N0, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", N0);
   // y inverse: columns (x, kz)
   a.in = p->bufA; a.out = p->bufB; a.tw = p->tw[N1];
   a.Q = N0 * Nh; a.cdiv = Nh; a.cvalid = Nh; a.in_hi = plane; a.out_hi = plane; a.in_row = Nh; a.out_row = Nh;
-  a.rin = rmap(N1 / 2, N1 / 2 - 1, N1); a.rout = a.rin;
+  a.rout = rmap(N1 / 2, N1 / 2 - 1, N1);
+  a.rin = trunc ? rmap(p->in[1][0], p->in[1][1], N1) : a.rout;
   if (launch_strided<Rt>(N1, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", N1);
   // z C2R
   ZArgs<Rt> z{};
   z.in = p->bufB; z.out = up; z.tw = p->tw[N2 / 2]; z.rtw = p->rtw[N2];
-  z.L = N0 * N1; z.nk = Nh; z.in_line = Nh; z.in_field = fs; z.out_line = N2 / 2;
+  z.L = N0 * N1; z.nk = trunc ? p->in[2][1] + 1 : Nh; z.in_line = Nh; z.in_field = fs; z.out_line = N2 / 2;
   z.out_field = (long)N0 * N1 * (N2 / 2); z.scale = Rt(1);
   if (launch_z<Rt>(N2 / 2, p->var[5], +1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", N2);
   return 0;
@@ -941,6 +947,17 @@ int sdns_backward(sdns_plan* p, const void* uh, void* up, int nc) {
   int rc = ensure_workspace(p);
   if (rc) return rc;
   rc = p->precision == SDNS_DOUBLE ? plain_backward<double>(p, uh, up, nc) : plain_backward<float>(p, uh, up, nc);
+  return rc ? rc : check_launch();
+}
+
+int sdns_backward_dealiased(sdns_plan* p, const void* uh, void* up, int nc) {
+  if (!p || !up || !uh || nc < 1) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (nc > 3) return fail(SDNS_ERR_INVALID, "ncomp %d > 3", nc);
+  if (p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = p->precision == SDNS_DOUBLE ? plain_backward<double>(p, uh, up, nc, true)
+                                   : plain_backward<float>(p, uh, up, nc, true);
   return rc ? rc : check_launch();
 }
 

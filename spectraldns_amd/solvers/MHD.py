@@ -6,8 +6,9 @@ The module keeps the reference's protocol (``get_context`` with its keys,
 ``set_Elsasser``, ``divergenceConvection``, ``getConvection('Divergence')``,
 ``add_pressure_diffusion`` with nu on U and eta on B, ``ComputeRHS``,
 ``get_divergence``) and RK4 runs stage by stage over ComputeRHS.  Each step
-of the RHS is a libsdns_hip launch: the 2/3-rule input truncation and the
-plain inverse transforms of the six fields, one kernel for the nine Elsasser
+of the RHS is a libsdns_hip launch: the inverse transforms of the six
+fields over their kept modes (the 2/3-rule truncation inside the transform
+passes), one kernel for the nine Elsasser
 products (u_i + b_i)(u_j - b_j), their forward transforms, one kernel for
 set_Elsasser, the Nyquist mask, the NS pressure/diffusion kernel on U and
 the diffusion kernel on B.
@@ -26,12 +27,11 @@ from ..fourier import FourierSpace, Array, Function
 
 class CompositeSpace:
     """ncomp scalar fields of one plan (shenfun CompositeSpace([T]*ncomp)).
-    Transforms run three components per launch; the dealiased space
-    truncates the inputs of its backward transform to the kept modes."""
+    Transforms run three components per launch sequence; the dealiased
+    space's backward transform reads only the kept modes."""
 
     def __init__(self, plan, ncomp, dealiased=False):
         self.plan, self.ncomp, self.dealiased = plan, ncomp, dealiased
-        self._tmp = None
 
     def shape(self, forward_output=False):
         return (self.ncomp,) + tuple(self.plan.spectral_shape if forward_output else self.plan.N)
@@ -42,14 +42,11 @@ class CompositeSpace:
         return u_hat
 
     def backward(self, u_hat, u):
-        import torch
         for c in range(0, u_hat.shape[0], 3):
-            src = u_hat[c:c + 3]
-            if self.dealiased and self.plan.dealias != "None":
-                if self._tmp is None:
-                    self._tmp = torch.empty_like(src)
-                src = self.plan.dealias_truncate(self._tmp, src)
-            self.plan.backward(src, u[c:c + 3])
+            if self.dealiased:
+                self.plan.backward_dealiased(u_hat[c:c + 3], u[c:c + 3])
+            else:
+                self.plan.backward(u_hat[c:c + 3], u[c:c + 3])
         return u
 
 

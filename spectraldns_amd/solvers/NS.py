@@ -131,11 +131,10 @@ def Cross(c, a, b, work, T):
     return T.forward(d, c)
 
 
-def _dealiased_backward(plan, u_hat, u, tmp):
-    """VTp.backward(u_hat) for the 2/3 rule or no dealiasing: input truncation
-    to the kept modes, then the inverse transform."""
-    src = u_hat if plan.dealias == "None" else plan.dealias_truncate(tmp, u_hat)
-    return plan.backward(src, u)
+def _dealiased_backward(plan, u_hat, u):
+    """VTp.backward(u_hat) for the 2/3 rule or no dealiasing: the inverse
+    transform of the kept modes (truncation inside the transform passes)."""
+    return plan.backward_dealiased(u_hat, u)
 
 
 def standard_convection(rhs, u_dealias, U_hat, work, Tp, K):
@@ -199,7 +198,7 @@ def getConvection(convection):
         plan = VTp.plan
         if plan.dealias == "3/2-rule":
             raise NotImplementedError("the %s form is built for the 2/3 rule and no dealiasing" % convection)
-        u_dealias = _dealiased_backward(plan, u_hat, u_dealias, work[(u_hat, 4, False)])
+        u_dealias = _dealiased_backward(plan, u_hat, u_dealias)
         if convection == "Standard":
             rhs = standard_convection(rhs, u_dealias, u_hat, work, Tp, K)
             plan.lincomb(rhs, [rhs], [-1.0])

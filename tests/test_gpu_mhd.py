@@ -95,6 +95,12 @@ def test_mhd_kernels(precision):
     kx, ky, kz = (np.abs(k) for k in g.kint)
     keep = (3 * kx[:, None, None] < N[0]) & (3 * ky[None, :, None] < N[1]) & (3 * kz[None, None, :] < N[2])
     assert np.array_equal(out.cpu().numpy(), np.where(keep, ZZ[:3], 0))
+    # the truncating inverse transform equals truncation + inverse transform, bit for bit
+    phys = torch.empty((3,) + N, dtype=zz.dtype, device="cuda")
+    p.backward(out, phys)
+    phys2 = torch.empty_like(phys)
+    p.backward_dealiased(a, phys2)
+    assert torch.equal(phys, phys2)
 
 
 def test_mhd_fp32_vs_fp64_oracle():

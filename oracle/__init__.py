@@ -10,4 +10,9 @@ baseline), never as the thing measured or shipped.
 ``solve`` loop of spectralDNS/__init__.py).  Each function cites the
 reference file:line it follows.  How each part is pinned (reference golden
 numbers, closed forms, brute-force sums) is stated in ``oracle/ns.py``.
+The other modules restate, on oracle.ns's transforms, the reference's other
+integrators (``oracle.integrators``), the velocity-vorticity solver
+(``oracle.vv``), the MHD solver (``oracle.mhd``), the NS Standard /
+Divergence / Skewed convection forms (``oracle.convection``) and the learned
+closure (``oracle.closure``); each module's header states its pins.
 """

@@ -127,3 +127,32 @@ def test_lincomb_and_error_norm_kernels(precision):
     sc = 1e-6 + np.maximum(np.abs(xs_np[4]), np.abs(xs_np[5])) * 1e-6
     ref = np.array([np.sum(np.abs(xs_np[3][k].astype(np.complex128) / sc[k]) ** 2) for k in range(3)])
     assert np.max(np.abs(nsq - ref) / ref) < (1e-13 if precision == "double" else 1e-5)
+
+
+def test_bs5_adaptive_rejection_path():
+    """A first step far too long for TOL is rejected and retried with a
+    shorter dt (integrators.py:127-138, FSAL offset rolled back); the run then
+    proceeds with the controller's steps.  GPU vs oracle: the same dt
+    sequence and the same final state (FP64, 1e-12)."""
+    from spectraldns_amd import config, get_solver, solve
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    N = (16, 16, 16)
+    u0 = random_solenoidal_field(N, seed=2, k0=3.0, energy=2.0)
+    g = Grid(N, (TWO_PI,) * 3, "double", "2/3-rule")
+    integ = OI.make("BS5_adaptive", g, 0.01, tol=1e-8)
+    ref, t, _, dts = OI.solve(integ, u0, 2.0, 1.0)
+    assert integ.rejected >= 1 and abs(t - 2.0) < 1e-12
+    times = []
+    config.update({"nu": 0.01, "dt": 1.0, "T": 2.0, "TOL": 1e-8, "dealias": "2/3-rule", "precision": "double"})
+    solver = get_solver(update=lambda c: times.append(config.params.t), regression_test=lambda c: None,
+                        parse_args=["--M", "4", "4", "4", "NS"])
+    config.params.integrator = "BS5_adaptive"
+    c = solver.get_context()
+    c.U_hat.copy_(torch.from_numpy(u0))
+    config.params.t = 0.0
+    config.params.tstep = 0
+    solve(solver, c)
+    config.update({"integrator": "RK4", "TOL": 1e-6})
+    assert len(times) == len(dts)
+    assert np.allclose(np.diff(np.concatenate([[0.0], times])), dts, rtol=1e-10, atol=0)
+    assert rel(c.U_hat.cpu().numpy(), ref) < 1e-12

@@ -151,7 +151,7 @@ def forced_main(args):
     def run(n):
         for s in range(n):
             plan.rk4_steps(u, [dt], nu, diag=diag[2 * s:], u_lo=ulo)
-            plan.force(u, target, kf)
+            plan.force(u, target, kf, u_lo=ulo)
 
     run(args.warmup)
     torch.cuda.synchronize()

@@ -28,6 +28,9 @@ def initialize(solver, context):
     u = random_solenoidal_field(config.params.N, config.params.L, seed=config.params.seed,
                                 k0=config.params.k0, energy=config.params.energy)
     c.U_hat.copy_(torch.from_numpy(u).to(c.U_hat.dtype))
+    if "VV" in config.params.solver:
+        # the state of the velocity-vorticity solver is the curl (Isotropic.py:71-72)
+        c.W_hat = solver.cross2(c.W_hat, c.K, c.U_hat)
     c.target_energy = energy_fourier(c.U_hat, c.T)
     config.params.t = 0.0
     config.params.tstep = 0
@@ -53,7 +56,15 @@ def update(context):
     c = context
     params = config.params
     solver = config.solver
-    c.T.plan.force(c.U_hat, c.target_energy, params.Kf2)
+    vv = "VV" in params.solver
+    if vv:
+        # velocity from the vorticity state before the forcing (Isotropic.py:160-161)
+        c.U_hat = solver.cross2(c.U_hat, c.K_over_K2, c.W_hat)
+    # the compensated single-precision state (params.compensated) is forced as a pair
+    c.T.plan.force(c.U_hat, c.target_energy, params.Kf2, u_lo=c.get("U_hat_lo"))
+    if vv:
+        # and the vorticity from the forced velocity (Isotropic.py:186-187)
+        c.W_hat = solver.cross2(c.W_hat, c.K, c.U_hat)
     if params.tstep % params.compute_spectrum == 0:
         Ek, bins = spectrum(c.U_hat, c.T)
         c.spectra[params.tstep] = Ek

@@ -137,6 +137,14 @@ int sdns_mask_nyquist(sdns_plan* plan, void* a_hat);
  * on the dealiased space, without mask, pressure or diffusion. */
 int sdns_convection(sdns_plan* plan, const void* u_hat, void* rhs);
 
+/* F_k(u_dealias x w_dealias) with the vorticity given: out = VTp.forward(
+ * VTp.backward(u_hat) x VTp.backward(w_hat)) on the dealiased space, without
+ * mask (Conv of the velocity-vorticity solver before its cross2,
+ * spectralDNS/solvers/VV.py:91-99: compute_velocity, VTp.backward(w_hat),
+ * Cross).  w_hat need not be a curl: no relation between u_hat and w_hat is
+ * assumed.  out may not alias u_hat or w_hat. */
+int sdns_convection_uw(sdns_plan* plan, const void* u_hat, const void* w_hat, void* out);
+
 /* ComputeRHS (NS.py:219-261) with the Vortex convection (NS.py:191-198):
  * dealiased u x curl(u), Nyquist mask, pressure and diffusion, + source
  * (source and p_hat may be NULL). */
@@ -158,6 +166,13 @@ int sdns_compute_rhs(sdns_plan* plan, const void* u_hat, void* rhs, double nu, c
  * reference leaves P_hat after a step. */
 int sdns_rk4_steps(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_seq, int nsteps, double nu,
                    double* diag, void* p_hat);
+/* sdns_rk4_steps with a Source added to every stage's right-hand side in
+ * every mode (rhs += Source after pressure and diffusion, NS.py:256-258, as
+ * the reference's RK4 calls ComputeRHS per stage, integrators.py:149-159).
+ * source: device array shaped like u_hat (NULL = none); held fixed through
+ * the steps, and it may not alias u_hat or u_lo. */
+int sdns_rk4_steps_source(sdns_plan* plan, void* u_hat, void* u_lo, const double* dt_seq, int nsteps, double nu,
+                          const void* source, double* diag, void* p_hat);
 
 /* Kinetic energy and dissipation of u_hat in FP64 into the device array out[2]. */
 int sdns_energy(sdns_plan* plan, const void* u_hat, double nu, double* out);
@@ -185,6 +200,12 @@ int sdns_error_norm(sdns_plan* plan, const void* err, const void* u0, const void
  * device (no host round trip).  sums (device, 2 doubles, may be NULL) receives
  * (energy_fourier before scaling, E_lower). */
 int sdns_force(sdns_plan* plan, void* u_hat, double target, double kf, double* sums);
+/* sdns_force on the compensated single-precision state u_hat + u_lo of
+ * sdns_rk4_steps (same forcing, Isotropic.py:154-184): the energies are FP64
+ * sums over u_hat + u_lo, and each forced mode is scaled as the FP64 value
+ * alpha*(u_hat + u_lo), then split again into u_hat = fl(value) and u_lo =
+ * the remainder.  SDNS_ERR_INVALID for a double-precision plan. */
+int sdns_force_compensated(sdns_plan* plan, void* u_hat, void* u_lo, double target, double kf, double* sums);
 
 /* Shell sums of the energy spectrum (demo/Isotropic.py:88-115): out (device,
  * 2*nbins doubles) receives, for shells z = 1..nbins, the sum over modes with

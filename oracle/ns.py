@@ -212,8 +212,9 @@ class Grid:
             rhs += source
         return rhs
 
-    def rk4_step(self, u0, dt, nu):
-        """One RK4 step (integrators.py:149-159, coefficients :185-186)."""
+    def rk4_step(self, u0, dt, nu, source=None):
+        """One RK4 step (integrators.py:149-159, coefficients :185-186); every
+        stage's ComputeRHS adds `source` (NS.py:256-258) when given."""
         a = np.array([1. / 6., 1. / 3., 1. / 3., 1. / 6.], dtype=self.float)
         b = np.array([0.5, 0.5, 1.], dtype=self.float)
         dt = self.float(dt)
@@ -221,7 +222,7 @@ class Grid:
         u1 = u0.copy()
         u2 = u0.copy()
         for rk in range(4):
-            rhs = self.compute_rhs(u0, nu)
+            rhs = self.compute_rhs(u0, nu, source)
             if rk < 3:
                 u0[:] = u1 + b[rk] * dt * rhs
             u2 += a[rk] * dt * rhs

@@ -48,8 +48,9 @@ def compute_rhs(g, w_hat, nu, source=None):
     return rhs
 
 
-def rk4_step(g, w0, dt, nu):
-    """RK4 (integrators.py:149-159) with the VV right-hand side."""
+def rk4_step(g, w0, dt, nu, source=None):
+    """RK4 (integrators.py:149-159) with the VV right-hand side; every stage
+    adds `source` when given."""
     a = np.array([1. / 6., 1. / 3., 1. / 3., 1. / 6.], dtype=g.float)
     b = np.array([0.5, 0.5, 1.], dtype=g.float)
     dt = g.float(dt)
@@ -57,7 +58,7 @@ def rk4_step(g, w0, dt, nu):
     w1 = w0.copy()
     w2 = w0.copy()
     for rk in range(4):
-        rhs = compute_rhs(g, w0, nu)
+        rhs = compute_rhs(g, w0, nu, source)
         if rk < 3:
             w0[:] = w1 + b[rk] * dt * rhs
         w2 += a[rk] * dt * rhs

@@ -48,10 +48,14 @@ _SIGS = {
     "sdns_compute_rhs": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
     "sdns_rk4_steps": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                       ctypes.c_double, _P, _P]),
+    "sdns_rk4_steps_source": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                             ctypes.c_double, _P, _P, _P]),
+    "sdns_convection_uw": (ctypes.c_int, [_P, _P, _P, _P]),
     "sdns_energy": (ctypes.c_int, [_P, _P, ctypes.c_double, _P]),
     "sdns_lincomb": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double)]),
     "sdns_error_norm": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_force": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, _P]),
+    "sdns_force_compensated": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
@@ -382,7 +386,16 @@ class Plan:
                                           _ptr(p_hat) if p_hat is not None else None))
         return rhs
 
-    def rk4_steps(self, u_hat, dts, nu, diag=None, u_lo=None, p_hat=None):
+    def convection_uw(self, u_hat, w_hat, out):
+        """out = F(u_dealias x w_dealias) with the vorticity w_hat given."""
+        self._spec(u_hat, 3, "u_hat")
+        self._spec(w_hat, 3, "w_hat")
+        self._spec(out, 3, "out")
+        self._sync_stream()
+        _check(self._lib.sdns_convection_uw(self._h, _ptr(u_hat), _ptr(w_hat), _ptr(out)))
+        return out
+
+    def rk4_steps(self, u_hat, dts, nu, diag=None, u_lo=None, p_hat=None, source=None):
         import torch
         self._spec(u_hat, 3, "u_hat")
         n = len(dts)
@@ -394,11 +407,14 @@ class Plan:
             self._spec(u_lo, 3, "u_lo")
         if p_hat is not None:
             self._spec(p_hat, 1, "P_hat")
+        if source is not None:
+            self._spec(source, 3, "Source")
         arr = (ctypes.c_double * max(n, 1))(*[float(d) for d in dts])
         self._sync_stream()
-        _check(self._lib.sdns_rk4_steps(self._h, _ptr(u_hat), _ptr(u_lo) if u_lo is not None else None,
-                                        arr, n, float(nu), _ptr(diag) if diag is not None else None,
-                                        _ptr(p_hat) if p_hat is not None else None))
+        _check(self._lib.sdns_rk4_steps_source(self._h, _ptr(u_hat), _ptr(u_lo) if u_lo is not None else None,
+                                               arr, n, float(nu), _ptr(source) if source is not None else None,
+                                               _ptr(diag) if diag is not None else None,
+                                               _ptr(p_hat) if p_hat is not None else None))
         return u_hat
 
     def lincomb(self, out, xs, cs):
@@ -423,8 +439,12 @@ class Plan:
 
     def error_norm(self, err, u0, u1, atol, rtol, out=None):
         """Per component sum of |err/sc|^2, sc = atol + max(|u0|, |u1|)*rtol, as a
-        float64 GPU tensor of 3 entries."""
+        float64 GPU tensor of 3 entries (3k entries for arrays of 3k components,
+        e.g. the six of the MHD state, as k launches of 3)."""
         import torch
+        if err.dim() == 4 and err.shape[0] > 3 and err.shape[0] % 3 == 0 and out is None:
+            return torch.cat([self.error_norm(err[c:c + 3], u0[c:c + 3], u1[c:c + 3], atol, rtol)
+                              for c in range(0, err.shape[0], 3)])
         for t, nm in ((err, "err"), (u0, "u0"), (u1, "u1")):
             self._spec(t, 3, nm)
         if out is None:
@@ -445,15 +465,21 @@ class Plan:
         _check(self._lib.sdns_energy(self._h, _ptr(u_hat), float(nu), _ptr(out)))
         return out
 
-    def force(self, u_hat, target, kf, sums=None):
-        """Isotropic.py:154-184 forcing in place (energy_fourier held at target)."""
+    def force(self, u_hat, target, kf, sums=None, u_lo=None):
+        """Isotropic.py:154-184 forcing in place (energy_fourier held at target).
+        With u_lo (single precision) the state is the compensated pair
+        u_hat + u_lo: energies of the pair, the pair scaled as one value."""
         import torch
         self._spec(u_hat, 3, "u_hat")
         if sums is not None and (sums.dtype != torch.float64 or sums.numel() < 2 or sums.device.type != "cuda"):
             raise SdnsError("sums must be a float64 GPU tensor of 2 entries")
+        sp = _ptr(sums) if sums is not None else None
         self._sync_stream()
-        _check(self._lib.sdns_force(self._h, _ptr(u_hat), float(target), float(kf),
-                                    _ptr(sums) if sums is not None else None))
+        if u_lo is not None:
+            self._spec(u_lo, 3, "u_lo")
+            _check(self._lib.sdns_force_compensated(self._h, _ptr(u_hat), _ptr(u_lo), float(target), float(kf), sp))
+        else:
+            _check(self._lib.sdns_force(self._h, _ptr(u_hat), float(target), float(kf), sp))
         return u_hat
 
     def spectrum_sums(self, u_hat, nbins):

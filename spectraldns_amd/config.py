@@ -126,18 +126,18 @@ parser.add_argument("--precision", default="double", choices=("single", "double"
 parser.add_argument("--optimization", default="hip",
                     choices=("hip", "", "cython", "weave", "numba", "pythran"),
                     help="Kernel backend (every choice runs the HIP kernels)")
-parser.add_argument("--make_profile", default=0, type=int, help="Enable cProfile profiler")
-parser.add_argument("--dt", default=0.01, type=float, help="Time step size")
-parser.add_argument("--T", default=0.1, type=float, help="End time")
+parser.add_argument("--make_profile", default=0, type=int, help="1: profile the run with cProfile")
+parser.add_argument("--dt", default=0.01, type=float, help="dt of the time loop (the last step may be shortened to end at T)")
+parser.add_argument("--T", default=0.1, type=float, help="simulated time at which the loop stops")
 parser.add_argument("--write_result", default=1e8, metavar=("tstep"), type=int,
-                    help="Write results every tstep")
-parser.add_argument("--checkpoint", default=1e8, type=int, help="Save intermediate result every...")
-parser.add_argument("--nu", default=0.000625, type=float, help="Viscosity")
-parser.add_argument("--t", default=0.0, type=float, help="Time")
-parser.add_argument("--tstep", default=0, type=int, help="Time step")
-parser.add_argument("--filemode", default="w", choices=("w", "r", "a"), help="File mode for results")
+                    help="steps between writes of the physical results")
+parser.add_argument("--checkpoint", default=1e8, type=int, help="steps between checkpoints of the spectral state")
+parser.add_argument("--nu", default=0.000625, type=float, help="kinematic viscosity nu")
+parser.add_argument("--t", default=0.0, type=float, help="simulated time at the start")
+parser.add_argument("--tstep", default=0, type=int, help="step counter at the start")
+parser.add_argument("--filemode", default="w", choices=("w", "r", "a"), help="open mode of the results files")
 parser.add_argument("--dealias", default="2/3-rule", choices=("2/3-rule", "3/2-rule", "None"),
-                    help="Choose dealiasing method")
+                    help="dealiasing of the nonlinear term: 2/3 truncation, 3/2 padding or none")
 parser.add_argument("--dealias_mode", default=0, type=int, choices=(0, 1),
                     help="2/3-rule variant: 0 truncate inputs |k| < N/3, 1 output mask |k| < 2/3(N/2+1)")
 parser.add_argument("--compensated", dest="compensated", action="store_true",
@@ -145,18 +145,18 @@ parser.add_argument("--compensated", dest="compensated", action="store_true",
 parser.set_defaults(compensated=False)
 parser.add_argument("--decomposition", default="slab", choices=("slab", "pencil"),
                     help="Kept for compatibility (one GPU holds the whole box)")
-parser.add_argument("--ntol", default=7, type=int, help="Tolerance - number of accurate digits")
+parser.add_argument("--ntol", default=7, type=int, help="decimal places the regression checks compare")
 parser.add_argument("--threads", default=1, type=int, help="Kept for compatibility")
 parser.add_argument("--planner_effort", action=PlanAction, default=fft_plans,
                     help="Kept for compatibility")
-parser.add_argument("--h5filename", default="results", type=str, help="Filename of stored results")
-parser.add_argument("--verbose", dest="verbose", action="store_true", help="Print timings in the end")
-parser.add_argument("--no-verbose", dest="verbose", action="store_false", help="Do not print timings")
+parser.add_argument("--h5filename", default="results", type=str, help="base name of the results files")
+parser.add_argument("--verbose", dest="verbose", action="store_true", help="report the step timings after the run")
+parser.add_argument("--no-verbose", dest="verbose", action="store_false", help="no timing report after the run")
 parser.set_defaults(verbose=True)
 parser.add_argument("--mask_nyquist", dest="mask_nyquist", action="store_true",
-                    help="Eliminate Nyquist frequency")
+                    help="zero the Nyquist planes of the right-hand side")
 parser.add_argument("--no-mask_nyquist", dest="mask_nyquist", action="store_false",
-                    help="Do not eliminate Nyquist frequency")
+                    help="keep the Nyquist planes of the right-hand side")
 parser.set_defaults(mask_nyquist=True)
 
 triplyperiodic = argparse.ArgumentParser(parents=[parser])
@@ -164,21 +164,21 @@ triplyperiodic.add_argument("--convection", default="Vortex",
                             choices=("Standard", "Divergence", "Skewed", "Vortex"),
                             help="Nonlinear term (the HIP path implements Vortex)")
 triplyperiodic.add_argument("--L", default=[2 * np.pi, 2 * np.pi, 2 * np.pi], metavar=("Lx", "Ly", "Lz"),
-                            nargs=3, help="Physical mesh size")
+                            nargs=3, help="box lengths (expressions in pi allowed, e.g. 2*pi)")
 triplyperiodic.add_argument("--M", default=[6, 6, 6], metavar=("Mx", "My", "Mz"), nargs=3,
-                            help="Mesh size is pow(2, M[i]) in direction i. Used if N is missing.")
-triplyperiodic.add_argument("--TOL", type=float, default=1e-6, help="Tolerance for adaptive integrators")
+                            help="log2 of the grid points per axis (N_i = 2**M_i) when N is not given")
+triplyperiodic.add_argument("--TOL", type=float, default=1e-6, help="error tolerance of the adaptive step-size control")
 triplyperiodic.add_argument("--integrator", default="RK4",
                             choices=("RK4", "ForwardEuler", "AB2", "BS5_adaptive", "BS5_fixed"),
-                            help="Integrator")
+                            help="time integration scheme")
 _sub = triplyperiodic.add_subparsers(dest="solver")
-_sub.add_parser("NS", help="Regular Navier Stokes solver")
-_sub.add_parser("VV", help="Velocity-Vorticity formulation")
+_sub.add_parser("NS", help="Navier-Stokes in velocity form")
+_sub.add_parser("VV", help="Navier-Stokes in velocity-vorticity form")
 _mhd = _sub.add_parser("MHD", help="Magnetohydrodynamics solver (2/3 rule or no dealiasing)")
-_mhd.add_argument("--eta", default=0.01, type=float, help="MHD parameter")
+_mhd.add_argument("--eta", default=0.01, type=float, help="magnetic diffusivity eta")
 _bq = _sub.add_parser("Bq", help="Boussinesq (not built)")
-_bq.add_argument("--Ri", default=0.1, type=float, help="Richardson number")
-_bq.add_argument("--Pr", default=1.0, type=float, help="Prandtl number")
+_bq.add_argument("--Ri", default=0.1, type=float, help="Richardson number Ri")
+_bq.add_argument("--Pr", default=1.0, type=float, help="Prandtl number Pr")
 
 
 def update(new, mesh="triplyperiodic"):

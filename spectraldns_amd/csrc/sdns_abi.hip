@@ -206,7 +206,8 @@ void compute_sizes(sdns_plan* p) {
   const size_t e = p->esz;
   const size_t M0 = p->M[0], M1 = p->M[1];
   const size_t state = 3ull * p->N[0] * p->N[1] * p->Nh;
-  const size_t w1 = 5ull * M0 * p->nky * p->nkzp, w3 = 3ull * M0 * M1 * p->nkzop;
+  // W1 holds 6 fields when the vorticity is given (sdns_convection_uw)
+  const size_t w1 = 6ull * M0 * p->nky * p->nkzp, w3 = 3ull * M0 * M1 * p->nkzop;
   const size_t w2 = 6ull * M0 * M1 * p->nkzp, w4 = 3ull * M0 * p->nkyo * p->nkzop;
   p->szA = align256(std::max(std::max(w1, w3), state) * e);
   p->szB = align256(std::max(std::max(w2, w4), state) * e);
@@ -355,13 +356,37 @@ template <typename Rt> StridedArgs<Rt> p4_args(const sdns_plan* p) {
   return a;
 }
 
-template <typename Rt> int conv_passes(sdns_plan* p, const void* u) {
+// P2 with a given vorticity: the plain y-inverse of the six W1 fields.
+template <typename Rt> StridedArgs<Rt> p2_given_args(const sdns_plan* p) {
   const int M0 = p->M[0], M1 = p->M[1];
+  StridedArgs<Rt> a{};
+  a.in = p->bufA; a.out = p->bufB; a.tw = p->tw.at(M1);
+  a.Q = M0 * p->nkzp; a.cdiv = p->nkzp; a.cvalid = p->nkz;
+  a.in_hi = (long)p->nky * p->nkzp; a.in_row = p->nkzp; a.in_field = (long)M0 * p->nky * p->nkzp;
+  a.out_hi = (long)M1 * p->nkzp; a.out_row = p->nkzp; a.out_field = (long)M0 * M1 * p->nkzp;
+  a.rin = rmap(p->in[1][0], p->in[1][1], p->nky);
+  a.rout = rmap(M1 / 2, M1 - M1 / 2 - 1, M1);
+  a.scale = Rt(1);
+  return a;
+}
+
+// P1..P4 of one convection evaluation from the state u into W4 (= bufB).  w:
+// the vorticity of u when given (the velocity-vorticity solver), otherwise
+// null and the passes form curl(u).
+template <typename Rt> int conv_passes(sdns_plan* p, const void* u, const void* w = nullptr) {
+  const int M0 = p->M[0], M1 = p->M[1];
+  P1Args<Rt> a1 = p1_args<Rt>(p, u);
+  a1.w = w;
   tstart(p, 0);
-  if (launch_p1<Rt>(M0, p->var[0], p1_args<Rt>(p, u), p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
+  if (launch_p1<Rt>(M0, p->var[0], a1, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
   tstop(p);
   tstart(p, 1);
-  if (launch_p2<Rt>(M1, p->var[1], p2_args<Rt>(p), p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  if (w) {
+    if (launch_strided<Rt>(M1, p->var[1], +1, 6, p2_given_args<Rt>(p), p->stream))
+      return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  } else if (launch_p2<Rt>(M1, p->var[1], p2_args<Rt>(p), p->stream)) {
+    return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  }
   tstop(p);
   tstart(p, 2);
   if (launch_p3<Rt>(p->n2h, p->var[2], p3_args<Rt>(p), p->stream))
@@ -400,9 +425,9 @@ template <typename Rt> int compute_rhs(sdns_plan* p, const void* u, void* rhs, d
   return 0;
 }
 
-template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs) {
+template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs, const void* w = nullptr) {
   int rc;
-  if ((rc = conv_passes<Rt>(p, u))) return rc;
+  if ((rc = conv_passes<Rt>(p, u, w))) return rc;
   P5Args<Rt> a = p5_args<Rt>(p, 0.0);
   a.A = const_cast<void*>(u); a.rhs = rhs; a.conv_only = 1;
   if (launch_p5<Rt>(p->M[0], p->var[4], P5_RHS, a, p->stream, nullptr))
@@ -446,8 +471,8 @@ int closure_source(sdns_closure* c, sdns_plan* p, const void* u, float in_scale,
 // held through the four stages (as the reference's Source, set between steps),
 // or, with per_stage, from every stage state.
 template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const double* dts, int nsteps, double nu,
-                                      double* diag, void* ph, sdns_closure* cl = nullptr, float in_scale = 1.f,
-                                      float out_scale = 1.f, int per_stage = 0) {
+                                      double* diag, void* ph, const void* src = nullptr, sdns_closure* cl = nullptr,
+                                      float in_scale = 1.f, float out_scale = 1.f, int per_stage = 0) {
   // a, b and dt in the working precision (integrators.py:185-186, config.py:123-127)
   const Rt a[4] = {Rt(1. / 6.), Rt(1. / 3.), Rt(1. / 3.), Rt(1. / 6.)};
   const Rt b[3] = {Rt(0.5), Rt(0.5), Rt(1.)};
@@ -462,7 +487,7 @@ template <typename Rt> int rk4_steps(sdns_plan* p, void* u, void* ulo, const dou
       P5Args<Rt> pa = p5_args<Rt>(p, nu);
       pa.A = u; pa.B = p->stB; pa.Cacc = p->stC;
       if (ulo) { pa.Alo = ulo; pa.Clo = p->stClo; }
-      if (cl) pa.source = cl->src;
+      pa.source = cl ? (const void*)cl->src : src;
       pa.adt = a[rk] * dt;
       pa.bdt = rk < 3 ? b[rk] * dt : Rt(0);
       const int mode = rk == 0 ? P5_RK0 : (rk < 3 ? P5_RK12 : P5_RK3);
@@ -523,10 +548,12 @@ template <typename Rt> int energy(sdns_plan* p, const void* u, double nu, double
   return 0;
 }
 
-template <typename Rt> int force(sdns_plan* p, void* u, double target, double kf, double* sums) {
+template <typename Rt> int force(sdns_plan* p, void* u, void* ulo, double target, double kf, double* sums) {
   EwArgs<Rt> a = ew_args<Rt>(p);
   a.a = u;
   a.out = u;
+  a.b = ulo;
+  a.aux = ulo;
   const Rt kf2 = (Rt)(kf * kf);
   double* s = sums ? sums : (double*)((char*)p->part + p->szPart - 4096);
   const int nblk = 1024;
@@ -1060,13 +1087,29 @@ int sdns_convection(sdns_plan* p, const void* u, void* rhs) {
   return rc ? rc : check_launch();
 }
 
-int sdns_rk4_steps(sdns_plan* p, void* u, void* u_lo, const double* dts, int nsteps, double nu, double* diag,
-                   void* ph) {
-  if (!p || !u || (!dts && nsteps > 0) || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
+int sdns_convection_uw(sdns_plan* p, const void* u, const void* w, void* out) {
+  if (!p || !u || !w || !out) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (out == u || out == w) return fail(SDNS_ERR_INVALID, "out may not alias an input");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = p->precision == SDNS_DOUBLE ? rk4_steps<double>(p, u, nullptr, dts, nsteps, nu, diag, ph)
-                                   : rk4_steps<float>(p, u, u_lo, dts, nsteps, nu, diag, ph);
+  rc = p->precision == SDNS_DOUBLE ? convection<double>(p, u, out, w) : convection<float>(p, u, out, w);
+  return rc ? rc : check_launch();
+}
+
+int sdns_rk4_steps(sdns_plan* p, void* u, void* u_lo, const double* dts, int nsteps, double nu, double* diag,
+                   void* ph) {
+  return sdns_rk4_steps_source(p, u, u_lo, dts, nsteps, nu, nullptr, diag, ph);
+}
+
+int sdns_rk4_steps_source(sdns_plan* p, void* u, void* u_lo, const double* dts, int nsteps, double nu,
+                          const void* source, double* diag, void* ph) {
+  if (!p || !u || (!dts && nsteps > 0) || nsteps < 0) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (source && (source == u || (u_lo && source == u_lo)))
+    return fail(SDNS_ERR_INVALID, "the source may not alias the state");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = p->precision == SDNS_DOUBLE ? rk4_steps<double>(p, u, nullptr, dts, nsteps, nu, diag, ph, source)
+                                   : rk4_steps<float>(p, u, u_lo, dts, nsteps, nu, diag, ph, source);
   return rc ? rc : check_launch();
 }
 
@@ -1074,7 +1117,17 @@ int sdns_force(sdns_plan* p, void* u, double target, double kf, double* sums) {
   if (!p || !u) return fail(SDNS_ERR_INVALID, "bad argument");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = p->precision == SDNS_DOUBLE ? force<double>(p, u, target, kf, sums) : force<float>(p, u, target, kf, sums);
+  rc = p->precision == SDNS_DOUBLE ? force<double>(p, u, nullptr, target, kf, sums)
+                                    : force<float>(p, u, nullptr, target, kf, sums);
+  return rc ? rc : check_launch();
+}
+
+int sdns_force_compensated(sdns_plan* p, void* u, void* u_lo, double target, double kf, double* sums) {
+  if (!p || !u || !u_lo) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (p->precision != SDNS_SINGLE) return fail(SDNS_ERR_INVALID, "the compensated state is single precision");
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  rc = force<float>(p, u, u_lo, target, kf, sums);
   return rc ? rc : check_launch();
 }
 
@@ -1398,7 +1451,7 @@ int sdns_closure_rk4_steps(sdns_closure* c, sdns_plan* p, void* u, const double*
     if (p->N[i] != c->n[i]) return fail(SDNS_ERR_INVALID, "plan and closure grids differ");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = rk4_steps<float>(p, u, nullptr, dts, nsteps, nu, diag, nullptr, c, in_scale, out_scale, per_stage);
+  rc = rk4_steps<float>(p, u, nullptr, dts, nsteps, nu, diag, nullptr, nullptr, c, in_scale, out_scale, per_stage);
   return rc ? rc : check_launch();
 }
 

@@ -98,7 +98,9 @@ __device__ __forceinline__ void table_to_lds(C* dst, const C* __restrict__ src, 
 
 // ------------------------------------------------------------------ P1
 // blockIdx.y selects the component: y = 0 -> w (fields 2 and 3 = X(kx w)),
-// y = 1 -> v (fields 1 and 4 = X(kx v)), y = 2 -> u (field 0).
+// y = 1 -> v (fields 1 and 4 = X(kx v)), y = 2 -> u (field 0).  With a given
+// vorticity (a.w) there are six blocks rows y = 0..5, each the plain
+// x-inverse of one field: u, v, w from a.u, then wx, wy, wz from a.w.
 template <typename Rt, int M, int E, int T>
 __device__ __forceinline__ void p1_body(const P1Args<Rt>& a, int tile, int yb, cx<Rt>* lds, const cx<Rt>* tw) {
   using C = cx<Rt>;
@@ -110,9 +112,11 @@ __device__ __forceinline__ void p1_body(const P1Args<Rt>& a, int tile, int yb, c
   const bool valid = q < a.Q && kzi < a.nkz;
   const int nky = a.y_neg + a.y_pos + 1;
   const int iy = iyc <= a.y_pos ? iyc : a.N1 - (nky - iyc);
-  const int comp = 2 - yb;
+  const bool given = a.w != nullptr;
+  const int comp = given ? yb % 3 : 2 - yb;
   const long plane = (long)a.N1 * a.Nh;
-  const C* __restrict__ U = (const C*)a.u + (long)comp * a.N0 * plane + (valid ? (long)iy * a.Nh + kzi : 0);
+  const C* __restrict__ U = (const C*)(given && yb >= 3 ? a.w : a.u) + (long)comp * a.N0 * plane +
+                            (valid ? (long)iy * a.Nh + kzi : 0);
   C v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -129,6 +133,11 @@ __device__ __forceinline__ void p1_body(const P1Args<Rt>& a, int tile, int yb, c
       for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * orow] = x[e];
     }
   };
+  if (given) {
+    F::run(v, t, line, T, tw);
+    store(yb, v);
+    return;
+  }
   if (comp > 0) {
     C x[E];
 #pragma unroll

@@ -10,7 +10,7 @@ namespace sdns {
 template <typename Rt, int M, int E> void p1_go(const P1Args<Rt>& a, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
-  hipLaunchKernelGGL((k_p1<Rt, M, E>), dim3(xcd_grid(nt), 3), dim3(Cfg::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_p1<Rt, M, E>), dim3(xcd_grid(nt), a.w ? 6 : 3), dim3(Cfg::THREADS), 0, s, a);
 }
 template <typename Rt, int M, int E> void p2_go(const P2Args<Rt>& a, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
@@ -471,10 +471,14 @@ __global__ void __launch_bounds__(256) k_force_reduce(EwArgs<Rt> a, Rt kf2, doub
     k2 += K1 * K1;
     k2 += K2 * K2;
     const C* A = (const C*)a.a;
-    const C u0 = A[i], u1 = A[n + i], u2 = A[2 * n + i];
+    const C* Lo = (const C*)a.b;  // low part of a compensated state, or null
+    double v[6] = {A[i].x, A[i].y, A[n + i].x, A[n + i].y, A[2 * n + i].x, A[2 * n + i].y};
+    if (Lo) {
+      v[0] += Lo[i].x; v[1] += Lo[i].y; v[2] += Lo[n + i].x;
+      v[3] += Lo[n + i].y; v[4] += Lo[2 * n + i].x; v[5] += Lo[2 * n + i].y;
+    }
     const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
-    const double e = w * ((double)u0.x * u0.x + (double)u0.y * u0.y + (double)u1.x * u1.x + (double)u1.y * u1.y +
-                          (double)u2.x * u2.x + (double)u2.y * u2.y);
+    const double e = w * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
     s_all += e;
     if (k2 <= kf2) s_low += e;
   }
@@ -500,8 +504,10 @@ __global__ void __launch_bounds__(256) k_force_scale(EwArgs<Rt> a, Rt kf2, const
   if (i >= a.n) return;
   const long n = a.n;
   C* U = (C*)a.out;
+  C* Lo = (C*)a.aux;  // low part of a compensated state, or null
   if (i == 0) {
     U[0] = czero<C>(); U[n] = czero<C>(); U[2 * n] = czero<C>();
+    if (Lo) { Lo[0] = czero<C>(); Lo[n] = czero<C>(); Lo[2 * n] = czero<C>(); }
     return;
   }
   const int kz = (int)(i % a.Nh);
@@ -514,6 +520,19 @@ __global__ void __launch_bounds__(256) k_force_scale(EwArgs<Rt> a, Rt kf2, const
   k2 += K2 * K2;
   if (!(k2 <= kf2)) return;
   const double lower = sums[1], upper = sums[0] - sums[1];
+  if (Lo) {
+    // the pair (hi, lo) scaled as one FP64 value, then split again
+    const double al = sqrt((target - upper) / lower);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long j = c * n + i;
+      const double x = al * ((double)U[j].x + (double)Lo[j].x), y = al * ((double)U[j].y + (double)Lo[j].y);
+      const C h = mk<C>((Rt)x, (Rt)y);
+      U[j] = h;
+      Lo[j] = mk<C>((Rt)(x - (double)h.x), (Rt)(y - (double)h.y));
+    }
+    return;
+  }
   const Rt alpha = (Rt)sqrt((target - upper) / lower);
   U[i] = cscale(U[i], alpha);
   U[n + i] = cscale(U[n + i], alpha);

@@ -29,7 +29,9 @@ struct RowMap {
 
 template <typename Rt> struct P1Args {  // x-inverse of u_hat + kx products
   const void* u;      // state [3][N0][N1][Nh]
-  void* w1;           // [5][M0][Q]
+  const void* w;      // given vorticity [3][N0][N1][Nh], or null: then W1 holds
+                      // the six plain x-inverses u, v, w, wx, wy, wz (no kx products)
+  void* w1;           // [5][M0][Q] ([6][M0][Q] with a given vorticity)
   const void* tw;     // forward twiddles of length M0
   const Rt* kx;       // wavenumbers of axis 0 by stored index
   int N0, N1, Nh;

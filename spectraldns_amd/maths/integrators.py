@@ -37,11 +37,12 @@ _BS5_BHAT = [2479 / 34992, 0, 123 / 416, 612941 / 3411720, 43 / 1440, 2272 / 656
 def RK4(u0, u1, u2, rhs, a, b, dt, solver, context):
     """One classical RK4 step on u0 in place; returns (u0, dt, dt).
 
+    Every stage adds context.Source, as ComputeRHS does (NS.py:256-258).
     With params.compensated in single precision the state is carried as
     u0 + context.U_hat_lo (a compensated pair, see sdns_rk4_steps)."""
     plan = context.VT.plan
     lo = context.get("U_hat_lo") if solver.params.get("compensated", False) else None
-    plan.rk4_steps(u0, [dt], solver.params.nu, p_hat=context.P_hat, u_lo=lo)
+    plan.rk4_steps(u0, [dt], solver.params.nu, p_hat=context.P_hat, u_lo=lo, source=context.Source)
     return u0, dt, dt
 
 
@@ -146,6 +147,9 @@ def getintegrator(rhs, u0, solver, context):
     params = solver.params
     if params.integrator not in ("RK4", "ForwardEuler", "AB2", "BS5_adaptive", "BS5_fixed"):
         raise NotImplementedError("integrator %r" % params.integrator)
+    if params.get("compensated", False) and params.integrator != "RK4":
+        raise NotImplementedError("the compensated state is carried by the fused NS RK4 step only "
+                                  "(integrator %r)" % params.integrator)
     if params.integrator == "RK4" and not (getattr(solver.conv, "fused_vortex", False) and
                                            solver.ComputeRHS is getattr(solver, "_ComputeRHS_fused", None)):
         if params.get("compensated", False):

@@ -222,13 +222,13 @@ def add_pressure_diffusion(rhs, u_hat, nu, K2, K, P_hat, K_over_K2):
 def ComputeRHS(rhs, u_hat, solver, **context):
     """rhs = conv(u_hat), Nyquist mask, pressure and diffusion, + Source (NS.py:219-261).
 
-    With the module's Vortex convection this is one fused launch sequence
-    (the Source is added when it was ever written).  A replaced ``solver.conv``
-    runs step by step with the same kernels."""
+    With the module's Vortex convection this is one fused launch sequence.
+    The Source is always added, as the reference does (it is zero unless the
+    user fills it, by any means).  A replaced ``solver.conv`` runs step by
+    step with the same kernels."""
     c = context
     VT = c["VT"]
-    Source = c["Source"]
-    src = Source if Source._version > 0 else None
+    src = c["Source"]
     if getattr(solver.conv, "fused_vortex", False):
         return VT.plan.compute_rhs(u_hat, rhs, params.nu, source=src, p_hat=c["P_hat"])
     u_dealias = c.get("u_dealias")
@@ -241,8 +241,7 @@ def ComputeRHS(rhs, u_hat, solver, **context):
     if c["mask"] is not None:
         VT.plan.mask_nyquist(rhs)
     rhs = solver.add_pressure_diffusion(rhs, u_hat, params.nu, None, c["K"], c["P_hat"], None)
-    if src is not None:
-        rhs += src
+    rhs += src
     return rhs
 
 

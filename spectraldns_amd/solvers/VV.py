@@ -6,14 +6,13 @@ without P and P_hat, with W_hat and u = W_hat), ``compute_velocity``,
 ``get_velocity``, ``get_curl``, ``get_divergence``, ``getConvection``,
 ``add_linear`` and ``ComputeRHS``.
 
-The Vortex convection F(u x w) runs on the fused NS passes: the velocity
-u_hat = 1j*(K/K^2) x W_hat (one kernel) goes through the dealiased
-convection launches, which form the vorticity as curl(u_hat).  For the
-solenoidal W_hat the solver carries (K . W_hat = 0 up to rounding, as W_hat
-starts as a curl and every RHS is one), curl(u_hat) = W_hat - K (K . W_hat)/K^2
-equals W_hat to rounding, so this is the reference's w_dealias =
-VTp.backward(w_hat).  K2 and K_over_K2 are handles that carry the plan; the
-kernels form K^2 on the fly, and indexing a handle materialises the array.
+The Vortex convection F(u x w) runs on the fused convection passes with the
+vorticity given: the velocity u_hat = 1j*(K/K^2) x W_hat (one kernel) and
+W_hat itself are transformed back together (sdns_convection_uw), so w is the
+reference's w_dealias = VTp.backward(w_hat) for any W_hat, including one with
+a mean or a gradient part (a random start, a Source with a mean).  K2 and
+K_over_K2 are handles that carry the plan; the kernels form K^2 on the fly,
+and indexing a handle materialises the array.
 """
 from .NS import *  # noqa: F401,F403
 from .NS import config, params, HDF5File, optimizer, cross2, end_of_tstep  # noqa: F401
@@ -120,13 +119,14 @@ def getConvection(convection):
 
 
 def _cross_convection(w_hat, work, VTp, K_over_K2):
-    """v_hat = F_k(u_dealias x w_dealias) with u_hat = 1j*(K/K^2) x w_hat (the
-    dealiased convection passes).  Both work arrays are written whole by the
-    next launch, so they are not zero-filled."""
+    """v_hat = F_k(u_dealias x w_dealias) with u_hat = 1j*(K/K^2) x w_hat and
+    w_dealias = VTp.backward(w_hat) (VV.py:95-97), in the dealiased
+    convection passes.  Both work arrays are written whole by the next
+    launch, so they are not zero-filled."""
     u_hat = work[(w_hat, 1, False)]
     u_hat = cross2(u_hat, K_over_K2, w_hat)
     v_hat = work[(w_hat, 0, False)]
-    return VTp.plan.convection(u_hat, v_hat)
+    return VTp.plan.convection_uw(u_hat, w_hat, v_hat)
 
 
 @optimizer
@@ -137,10 +137,9 @@ def add_linear(rhs, w_hat, nu, K2, Source):
 def ComputeRHS(rhs, w_hat, solver, **context):
     """rhs = conv(w_hat), Nyquist mask, -nu K^2 w_hat, + Source (VV.py:117-146).
 
-    The Source is added when it was ever written (it is zero otherwise)."""
+    The Source is always added, as the reference does."""
     c = context
-    Source = c["Source"]
-    src = Source if Source._version > 0 else None
+    src = c["Source"]
     if getattr(solver.conv, "vv_fused_tail", False) and solver.add_linear is add_linear:
         # the module's own Conv and add_linear: cross2, mask and add_linear in one launch
         v_hat = _cross_convection(w_hat, c["work"], c["VTp"], c["K_over_K2"])

@@ -49,23 +49,23 @@ def datatypes(precision):
 
 
 def regression_test(context):
-    """Optional function called at the end"""
+    """Hook run once after the last time step (a no-op unless the user sets one)."""
 
 
 def update(context):
-    """Optional function called every time step"""
+    """Hook run after each time step (a no-op unless the user sets one)."""
 
 
 def additional_callback(context):
-    """Function used by some integrators"""
+    """Hook the adaptive integrator calls after a stage or a rejected step."""
 
 
 def solve_linear(context):
-    """Function used by implicit solvers"""
+    """Linear-solve hook of implicit schemes (none of the built solvers needs it)."""
 
 
 def conv(*args):
-    """Function used to compute convective term"""
+    """Placeholder for the nonlinear term; solve() installs getConvection's result."""
     raise NotImplementedError
 
 

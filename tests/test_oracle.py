@@ -237,6 +237,83 @@ def test_spectrum_single_shell():
     assert np.count_nonzero(Ek > 1e-12 * Ek.max()) == 1 and Ek[1] == Ek.max()
 
 
+def brute_force_spectrum(N, L, uh):
+    """Shell spectrum by an explicit loop over every stored mode, written from
+    the definition (demo/Isotropic.py:88-115), with no digitize and no
+    vectorised weights: mode (i, j, l) has wavenumber
+    k = (2 pi/L_a) * (index, or index - N_a above N_a/2) on axes 0 and 1 and
+    (2 pi/L_2) * l on the R2C axis; its weight is 1 on l = 0 and l = N2/2 and 2
+    elsewhere; shell s = 1..Nb-1 holds s - 1/2 < |k| <= s + 1/2, and
+    E(s) = (4 pi/3) ((s + 1/2)^3 - (s - 1/2)^3) * (sum of w sum_c |u_c|^2) / count."""
+    import math
+    N0, N1, N2 = N
+    Nh = N2 // 2 + 1
+    Nb = int(math.sqrt(sum((n / 2.0) ** 2 for n in N) / 3))
+    sums = [0.0] * Nb
+    counts = [0] * Nb
+
+    def wn(i, n, length):
+        return 2 * math.pi / length * (i if 2 * i < n else i - n)
+
+    for i in range(N0):
+        kx = wn(i, N0, L[0])
+        for j in range(N1):
+            ky = wn(j, N1, L[1])
+            for m in range(Nh):
+                kz = 2 * math.pi / L[2] * m
+                k = math.sqrt(kx * kx + ky * ky + kz * kz)
+                w = 1.0 if m == 0 or m == N2 // 2 else 2.0
+                e = w * sum(abs(complex(uh[c, i, j, m])) ** 2 for c in range(3))
+                for s in range(1, Nb):
+                    if s - 0.5 < k <= s + 0.5:
+                        sums[s - 1] += e
+                        counts[s - 1] += 1
+    Ek = [0.0] * Nb
+    for s in range(1, Nb):
+        if counts[s - 1]:
+            Ek[s - 1] = 4 * math.pi / 3 * ((s + 0.5) ** 3 - (s - 0.5) ** 3) * sums[s - 1] / counts[s - 1]
+    return np.array(Ek), np.array(counts)
+
+
+@pytest.mark.parametrize("N,L", [((8, 8, 8), (TWO_PI,) * 3),
+                                 ((12, 8, 16), (TWO_PI, TWO_PI, TWO_PI)),
+                                 ((8, 12, 10), (TWO_PI, 2 * TWO_PI, 0.5 * TWO_PI))])
+def test_spectrum_brute_force(N, L):
+    """Every shell of a random multi-shell field against the explicit loop:
+    the 4 pi/3 factor, the (k^3 - k0^3)/count normalisation, the kz weights
+    1/2/1 and the scaled wavenumbers of a non-2pi box all enter."""
+    rng = np.random.default_rng(3)
+    g = Grid(N, L)
+    uh = g.forward(rng.standard_normal((3,) + N))
+    Ek, bins = g.spectrum(uh)
+    ref, counts = brute_force_spectrum(N, L, uh)
+    assert np.count_nonzero(ref) >= 3
+    assert np.allclose(bins, np.arange(len(bins)) + 0.5)
+    assert np.max(np.abs(Ek - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_spectrum_closed_form():
+    """A single Fourier mode (kx, ky, kz) = (2, 1, 3) of amplitude a in one
+    component, with its conjugate partner on the kz = 0 plane absent (kz > 0):
+    |k| = sqrt(14) lies in shell 4 (3.5, 4.5], whose modes number exactly the
+    lattice points counted by hand below; E(4) = 4pi/3 (4.5^3 - 3.5^3) 2|a|^2 / n4."""
+    N = (16, 16, 16)
+    g = Grid(N)
+    uh = np.zeros((3, 16, 16, 9), dtype=np.complex128)
+    a = 0.3 - 0.4j
+    uh[1, 2, 1, 3] = a
+    Ek, _ = g.spectrum(uh)
+    n4 = 0
+    for i in range(-8, 8):
+        for j in range(-8, 8):
+            for m in range(0, 9):
+                if 3.5 ** 2 < i * i + j * j + m * m <= 4.5 ** 2:
+                    n4 += 1
+    expect = 4 * np.pi / 3 * (4.5 ** 3 - 3.5 ** 3) * 2 * abs(a) ** 2 / n4
+    assert abs(Ek[3] - expect) <= 1e-15 * expect
+    assert np.count_nonzero(Ek) == 1
+
+
 # ------------------------------------------------------------------ time loop
 def test_dt_schedule_last_step():
     d = dt_schedule(0.1, 0.01)

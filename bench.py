@@ -201,14 +201,88 @@ def measured_traffic(N, dealias, dom, variants):
             "source": "profiles/" + t["source"]}
 
 
-def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(N, precision, dealias, nu=0.001, dt=0.001, warmup=1, timed=5):
+    """The NumPy/SciPy oracle on this host's cores: `warmup` untimed RK4 steps,
+    then the median wall time of `timed` RK4 steps of the same workload."""
     from oracle.ns import Grid, _WORKERS
     from spectraldns_amd.isotropic import random_solenoidal_field
     g = Grid(N, (2 * np.pi,) * 3, precision, dealias)
     u = random_solenoidal_field(N, seed=1234, k0=4.0, energy=0.5).astype(g.complex)
-    t0 = time.perf_counter()
-    g.rk4_step(u, dt, nu)
-    return time.perf_counter() - t0, _WORKERS
+    for _ in range(warmup):
+        u = g.rk4_step(u, dt, nu)
+    times = []
+    for _ in range(timed):
+        t0 = time.perf_counter()
+        u = g.rk4_step(u, dt, nu)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), _WORKERS, times
+
+
+# Closure experiment figures of the committed summary
+# profiles/closure_64cube_from_256cube_fp32.md (demo/Closure.py run: 256^3 forced
+# DNS filtered by 4, 2000 coarse steps); quoted, not re-measured by bench.py.
+CLOSURE_SUMMARY = {
+    "source": "profiles/closure_64cube_from_256cube_fp32.md",
+    "spectrum_error_time_averaged": 0.045, "spectrum_error_instantaneous_mean": 0.089,
+    "dissipation_error_time_averaged": 0.071, "dissipation_error_instantaneous_mean": 0.078,
+    "no_model_spectrum_error_time_averaged": 0.231,
+}
+
+
+def closure_timing(steps=20, warmup=3):
+    """GPU time per step of the two runs the closure metric compares, measured
+    here with HIP events on the solver stream: the 256^3 forced DNS (FP32, 2/3
+    rule, nu = 0.0008, dt = 0.002, RK4 + forcing) and the 64^3 coarse run with
+    the learned closure (FP32, 2/3 rule with the output mask, dt = 0.008, RK4
+    whose stages add the network's Source, + forcing).  The network's cost
+    does not depend on its weights, so canonical initial weights are used.
+    ratio = GPU time of the DNS / GPU time of the coarse run over the same
+    simulated time (4 DNS steps per coarse step)."""
+    import torch
+    from spectraldns_amd import closure as CL
+
+    def per_step(run, n_warm, n):
+        for _ in range(n_warm):
+            run.step()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            run.step()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n
+
+    from spectraldns_amd._abi import Plan
+    from spectraldns_amd.isotropic import random_solenoidal_field_gpu
+    u0 = random_solenoidal_field_gpu(Plan((256,) * 3, (2 * np.pi,) * 3, "single", "2/3-rule"), seed=1234)
+    dns = CL.ForcedRun((256,) * 3, 0.0008, 0.002, u_hat=u0)
+    ms_dns = per_step(dns, warmup, steps)
+    del dns
+    torch.cuda.empty_cache()
+    net = CL.Closure((64,) * 3)
+    coarse = CL.ForcedRun((64,) * 3, 0.0008, 0.008, seed=1234, closure=net, dealias_mode=1)
+    ms_les = per_step(coarse, warmup, 5 * steps)
+    plain = CL.ForcedRun((64,) * 3, 0.0008, 0.008, seed=1234, dealias_mode=1)
+    ms_plain = per_step(plain, warmup, 5 * steps)
+    ratio = (ms_dns * 4.0) / ms_les
+    out = {"dns_256cube_ms_per_step": round(ms_dns, 4), "dns_dt": 0.002,
+           "les_64cube_ms_per_step": round(ms_les, 4), "les_dt": 0.008,
+           "les_64cube_no_model_ms_per_step": round(ms_plain, 4),
+           "gpu_time_ratio_dns_over_les": round(ratio, 2),
+           "errors_quoted": dict(CLOSURE_SUMMARY)}
+    return out
 
 
 # Algorithmic HBM bytes per time step of the solver workloads, in units of
@@ -306,6 +380,7 @@ def main():
     ap.add_argument("--dealias", default="3/2-rule", choices=("3/2-rule", "2/3-rule"))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-closure", action="store_true", help="skip the closure/DNS step timing")
     ap.add_argument("--workload", default="isotropic", choices=("isotropic", "ensemble", "forced", "solver"),
                     help="isotropic: the 256^3 box (default); ensemble: the 96-box uneven ensemble; "
                          "forced: forced isotropic turbulence with compensated FP32 (use --N 1024); "
@@ -388,11 +463,16 @@ def main():
                        "step_roofline_frac": round(b64 * sps64 / 1e9 / HBM_PEAK_GBS, 4),
                        "passes": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1]}
                                   for k, v in pass64.items()}}
+    if not args.no_closure:
+        out["closure"] = closure_timing()
     if not args.no_cpu:
-        t_cpu, cores = cpu_baseline(N, "single", args.dealias)
+        t_cpu, cores, times = cpu_baseline(N, "single", args.dealias)
         out["cpu_baseline"] = {"value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": cores, "kind": "port",
-                               "sample": "1 RK4 step of the same %d^3 %s FP32 workload with the NumPy/SciPy "
-                                         "oracle (scipy.fft workers=%d)" % (args.N, args.dealias, cores)}
+                               "cpu_model": cpu_model(),
+                               "step_s": [round(t, 3) for t in times],
+                               "sample": "1 warm-up + median of 5 RK4 steps of the same %d^3 %s FP32 workload "
+                                         "with the NumPy/SciPy oracle (scipy.fft workers=%d)"
+                                         % (args.N, args.dealias, cores)}
     print(json.dumps(out), flush=True)
 
 

@@ -212,6 +212,16 @@ int sdns_force_compensated(sdns_plan* plan, void* u_hat, void* u_lo, double targ
  * z - 1/2 < |K| <= z + 1/2 of uiui = sum_c w |u_hat_c|^2 (out[z-1]) and the
  * number of such modes (out[nbins + z - 1]). */
 int sdns_spectrum(sdns_plan* plan, const void* u_hat, int nbins, double* out);
+/* sdns_spectrum that allocates nothing: no counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight diagnostics loops).  The per-block partial sums go to a
+ * static device array of the library (4 MiB on every GPU it runs on, held
+ * until the process ends) instead of the plan's workspace, so the plan's
+ * workspace is not allocated by this call.  nbins is at most 8192
+ * (SDNS_ERR_INVALID beyond).  Same results as sdns_spectrum up to the order
+ * of the FP64 additions.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int sdns_spectrum_noalloc(sdns_plan* plan, const void* u_hat, int nbins, double* out);
 
 /* Per-pass GPU time from HIP events recorded on the plan's stream around
  * every pass launch (benchmark support).  sdns_timing_enable(plan, 1) starts

@@ -57,6 +57,7 @@ _SIGS = {
     "sdns_force": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_force_compensated": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "sdns_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
+    "sdns_spectrum_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sdns_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "sdns_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "sdns_timing_passes": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -482,14 +483,21 @@ class Plan:
             _check(self._lib.sdns_force(self._h, _ptr(u_hat), float(target), float(kf), sp))
         return u_hat
 
-    def spectrum_sums(self, u_hat, nbins):
-        """(uiui sums, mode counts) per shell z = 1..nbins, as float64 GPU tensors."""
+    def spectrum_sums(self, u_hat, nbins, out=None, noalloc=False):
+        """(uiui sums, mode counts) per shell z = 1..nbins, as float64 GPU tensors
+        (views of ``out``, 2*nbins doubles, when given).  ``noalloc`` calls
+        sdns_spectrum_noalloc, whose partial sums live in the library's static
+        scratch instead of the plan's workspace."""
         import torch
         self._spec(u_hat, 3, "u_hat")
-        out = torch.empty(2 * nbins, dtype=torch.float64, device=self.device)
+        if out is None:
+            out = torch.empty(2 * nbins, dtype=torch.float64, device=self.device)
+        elif out.dtype != torch.float64 or out.numel() < 2 * nbins or out.device.type != "cuda":
+            raise SdnsError("out must be a float64 GPU tensor of 2*nbins entries")
         self._sync_stream()
-        _check(self._lib.sdns_spectrum(self._h, _ptr(u_hat), int(nbins), _ptr(out)))
-        return out[:nbins], out[nbins:]
+        fn = self._lib.sdns_spectrum_noalloc if noalloc else self._lib.sdns_spectrum
+        _check(fn(self._h, _ptr(u_hat), int(nbins), _ptr(out)))
+        return out[:nbins], out[nbins:2 * nbins]
 
     def box_filter(self, u, r):
         """(ubar (3, N/r...), tau (6, N/r...)) of the physical velocity u (block means, FP64 sums)."""

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -563,15 +564,20 @@ template <typename Rt> int force(sdns_plan* p, void* u, void* ulo, double target
   return 0;
 }
 
-template <typename Rt> int spectrum(sdns_plan* p, const void* u, int nb, double* out) {
+// Shell sums through `part` (room for nblk * 2 * nb doubles): every block
+// writes its whole row of partials, so part needs no clearing.
+template <typename Rt> int spectrum(sdns_plan* p, const void* u, int nb, double* part, int nblk, double* out) {
   EwArgs<Rt> a = ew_args<Rt>(p);
   a.a = u;
+  launch_spectrum<Rt>(a, nb, part, nblk, p->stream);
+  if (launch_sum_rows(part, nblk, 2 * nb, out, p->stream)) return fail(SDNS_ERR_HIP, "sum rows");
+  return 0;
+}
+template <typename Rt> int spectrum(sdns_plan* p, const void* u, int nb, double* out) {
   const size_t avail = p->szPart - 4096;
   int nblk = (int)std::min<size_t>(256, avail / (2 * (size_t)nb * sizeof(double)));
   if (nblk < 1) return fail(SDNS_ERR_INVALID, "nbins %d too large", nb);
-  launch_spectrum<Rt>(a, nb, p->part, nblk, p->stream);
-  if (launch_sum_rows(p->part, nblk, 2 * nb, out, p->stream)) return fail(SDNS_ERR_HIP, "sum rows");
-  return 0;
+  return spectrum<Rt>(p, u, nb, p->part, nblk, out);
 }
 
 int check_launch() {
@@ -867,6 +873,41 @@ int batch_capture(sdns_batch* b) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free spectrum
+// Scratch of sdns_spectrum_noalloc: the per-block shell partials, 2^19 doubles
+// (4 MiB) on every GPU the library runs on, held until the process ends.  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// kernels get the address that hipGetSymbolAddress returns for the current
+// device (scratch_on_device).
+namespace sdns {
+constexpr int kScratchDoubles = 1 << 19;
+constexpr int kNoallocMaxBins = 8192;  // 2 * nbins doubles of LDS per block: 128 KiB
+__device__ __align__(256) double scratch[kScratchDoubles];
+}  // namespace sdns
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_device(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SDNS_ERR_UNSUPPORTED, "device %d (at most %d)", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(sdns::scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SDNS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = (double*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char* sdns_last_error(void) { return g_err.c_str(); }
@@ -1136,6 +1177,20 @@ int sdns_spectrum(sdns_plan* p, const void* u, int nbins, double* out) {
   int rc = ensure_workspace(p);
   if (rc) return rc;
   rc = p->precision == SDNS_DOUBLE ? spectrum<double>(p, u, nbins, out) : spectrum<float>(p, u, nbins, out);
+  return rc ? rc : check_launch();
+}
+
+int sdns_spectrum_noalloc(sdns_plan* p, const void* u, int nbins, double* out) {
+  if (!p || !u || !out) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (nbins < 1 || nbins > kNoallocMaxBins)
+    return fail(SDNS_ERR_INVALID, "nbins %d outside 1..%d (the limit of the no-alloc scratch of %d doubles)", nbins,
+                kNoallocMaxBins, kScratchDoubles);
+  double* part = nullptr;
+  int rc = scratch_on_device(&part);
+  if (rc) return rc;
+  const int nblk = std::min(256, kScratchDoubles / (2 * nbins));  // >= 32
+  rc = p->precision == SDNS_DOUBLE ? spectrum<double>(p, u, nbins, part, nblk, out)
+                                   : spectrum<float>(p, u, nbins, part, nblk, out);
   return rc ? rc : check_launch();
 }
 

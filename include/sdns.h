@@ -95,12 +95,14 @@ int sdns_add_linear(sdns_plan* plan, void* rhs, const void* w_hat, double nu, co
  * when not NULL (add_linear, VV.py:110-115); v_hat = F(u x w). */
 int sdns_vv_rhs_tail(sdns_plan* plan, void* rhs, const void* v_hat, const void* w_hat, double nu,
                      const void* source);
-/* MHD solver in Elsasser form (spectralDNS/solvers/MHD.py), 2/3 rule or no
- * dealiasing:
+/* MHD solver in Elsasser form (spectralDNS/solvers/MHD.py), any dealiasing
+ * (the 3/2 rule runs the products on the padded grid of
+ * sdns_backward_dealiased / sdns_forward_dealiased):
  *  dealias_truncate: out = a with the modes outside the plan's kept input
  *    range zeroed, 3 components (the input truncation of shenfun's dealiased
  *    backward transform with dealias_direct, as VMp.backward(ub_hat) in
- *    Conv, MHD.py:140-146); SDNS_ERR_UNSUPPORTED for a 3/2-rule plan
+ *    Conv, MHD.py:140-146); for a 3/2-rule plan the kept input modes are
+ *    |k| < N/2 (the Nyquist modes zeroed)
  *  elsasser_products: zz[3i+j] = (u_i + b_i)*(u_j - b_j) on npoints physical
  *    points per component, ub = (u0, u1, u2, b0, b1, b2) (divergenceConvection,
  *    MHD.py:118-130, before its forward transforms)
@@ -108,18 +110,31 @@ int sdns_vv_rhs_tail(sdns_plan* plan, void* rhs, const void* v_hat, const void* 
  *    c_hat[3:6] = 1j*sum_j K_j (zz_ji - zz_ij)/2 from the 9 spectral
  *    products (set_Elsasser, MHD.py:104-113) */
 int sdns_dealias_truncate(sdns_plan* plan, void* out, const void* a);
-/* dealias_truncate followed by sdns_backward, in the transform's own passes
- * (each pass reads only the kept modes of its axis): VMp.backward / VTp.backward
- * of shenfun's 2/3 rule (dealias_direct) or of no dealiasing; ncomp <= 3;
- * SDNS_ERR_UNSUPPORTED for a 3/2-rule plan. */
+/* Backward transform of the dealiased space Tp / VTp / VMp (NS.py:29-32,
+ * MHD.py:30-32; VMp.backward in Conv, MHD.py:140-146), ncomp <= 3:
+ *  - 2/3 rule (dealias_direct) and no dealiasing: dealias_truncate followed
+ *    by sdns_backward, in the transform's own passes (each pass reads only the
+ *    kept modes of its axis); u is (ncomp, N0, N1, N2);
+ *  - 3/2 rule: the padded transform; u is (ncomp, M0, M1, M2), M = 3N/2, the
+ *    modes |k| < N/2 of every axis placed in the padded spectrum (the Nyquist
+ *    modes dropped), the other bins zero. */
 int sdns_backward_dealiased(sdns_plan* plan, const void* u_hat, void* u, int ncomp);
+/* Forward transform of the dealiased space (Tp.forward in divergenceConvection,
+ * MHD.py:118-130, and the Standard/Divergence/Skewed forms, NS.py:138-176),
+ * ncomp <= 3: sdns_forward for the 2/3 rule and no dealiasing; for the 3/2
+ * rule the forward transform of the padded grid (ncomp, M0, M1, M2), scaled by
+ * 1/(M0 M1 M2), truncated to the modes -N/2..N/2-1 (axes 0, 1) and 0..N/2
+ * (axis 2) of u_hat. */
+int sdns_forward_dealiased(sdns_plan* plan, const void* u, void* u_hat, int ncomp);
 int sdns_elsasser_products(sdns_plan* plan, void* zz, const void* ub, size_t npoints);
 int sdns_set_elsasser(sdns_plan* plan, void* c_hat, const void* zz_hat);
 /* The Standard, Divergence and Skewed convection forms of the NS solver
- * (spectralDNS/solvers/NS.py:138-199), 2/3 rule or no dealiasing:
+ * (spectralDNS/solvers/NS.py:138-199), any dealiasing (physical arrays on the
+ * plan's grid, or on the padded grid for the 3/2 rule):
  *  gradient_truncated: out[j] = 1j*K_j*a on the kept modes, 0 elsewhere, for
  *    one spectral component a (the input of Tp.backward(1j*K[j]*U_hat[i]) in
- *    standard_convection, NS.py:138-145); SDNS_ERR_UNSUPPORTED for 3/2
+ *    standard_convection, NS.py:138-145); the kept modes are those of
+ *    sdns_dealias_truncate
  *  dot3: out = u0*g0 + u1*g1 + u2*g2 on npoints physical points
  *    (np.sum(u_dealias*gradUi, 0))
  *  uu_products: uu = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) on npoints

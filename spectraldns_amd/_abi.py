@@ -37,6 +37,7 @@ _SIGS = {
     "sdns_vv_rhs_tail": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, _P]),
     "sdns_dealias_truncate": (ctypes.c_int, [_P, _P, _P]),
     "sdns_backward_dealiased": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
+    "sdns_forward_dealiased": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
     "sdns_elsasser_products": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "sdns_set_elsasser": (ctypes.c_int, [_P, _P, _P]),
     "sdns_gradient_truncated": (ctypes.c_int, [_P, _P, _P]),
@@ -155,6 +156,8 @@ class Plan:
         self.workspace_bytes = nbytes
         self.spectral_shape = (self.N[0], self.N[1], self.N[2] // 2 + 1)
         self.physical_shape = self.N
+        # physical grid of the dealiased space: padded (3N/2) for the 3/2 rule
+        self.dealiased_shape = tuple(3 * n // 2 for n in self.N) if dealias == "3/2-rule" else self.N
         from .autotune import apply_schedule
         self.schedule = apply_schedule(self) if tuned else None
 
@@ -179,11 +182,12 @@ class Plan:
             raise SdnsError("%s must have %s components" % (name, ncomp))
         return nc
 
-    def _phys(self, t, name="array"):
+    def _phys(self, t, name="array", shape=None):
+        shape = self.physical_shape if shape is None else shape
         if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous():
             raise SdnsError("%s must be a contiguous %s tensor on the GPU" % (name, self.real_dtype))
-        if tuple(t.shape[-3:]) != self.physical_shape or t.dim() not in (3, 4):
-            raise SdnsError("%s has shape %s, expected (..., %d, %d, %d)" % ((name, tuple(t.shape)) + self.physical_shape))
+        if tuple(t.shape[-3:]) != shape or t.dim() not in (3, 4):
+            raise SdnsError("%s has shape %s, expected (..., %d, %d, %d)" % ((name, tuple(t.shape)) + shape))
         return 1 if t.dim() == 3 else t.shape[0]
 
     # ---------------------------------------------------------------- entry points
@@ -204,14 +208,25 @@ class Plan:
         return u
 
     def backward_dealiased(self, u_hat, u):
-        """Inverse transform of the kept input modes only (the 2/3-rule or
-        undealiased VTp.backward), the truncation done inside the passes."""
+        """VTp.backward: the inverse transform of the kept input modes only (2/3
+        rule or no dealiasing, the truncation done inside the passes), or the
+        padded inverse transform onto the 3N/2 grid ``dealiased_shape`` (3/2 rule)."""
         nc = self._spec(u_hat, None, "u_hat")
-        if self._phys(u, "u") != nc:
+        if self._phys(u, "u", self.dealiased_shape) != nc:
             raise SdnsError("component mismatch")
         self._sync_stream()
         _check(self._lib.sdns_backward_dealiased(self._h, _ptr(u_hat), _ptr(u), nc))
         return u
+
+    def forward_dealiased(self, u, u_hat):
+        """Tp.forward: sdns_forward (2/3 rule, no dealiasing), or the forward
+        transform of the padded grid truncated to the plan's modes (3/2 rule)."""
+        nc = self._phys(u, "u", self.dealiased_shape)
+        if self._spec(u_hat, nc, "u_hat") != nc:
+            raise SdnsError("component mismatch")
+        self._sync_stream()
+        _check(self._lib.sdns_forward_dealiased(self._h, _ptr(u), _ptr(u_hat), nc))
+        return u_hat
 
     def cross1(self, c, a, b):
         for t, nm in ((c, "c"), (a, "a"), (b, "b")):
@@ -250,7 +265,8 @@ class Plan:
         """zz[3i+j] = (u_i + b_i)(u_j - b_j) on physical (6, ...) -> (9, ...) arrays."""
         for t, nm, nc in ((zz, "zz", 9), (ub, "ub", 6)):
             if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous() \
-                    or tuple(t.shape) != (nc,) + self.physical_shape:
+                    or tuple(t.shape) not in ((nc,) + self.physical_shape, (nc,) + self.dealiased_shape) \
+                    or tuple(t.shape[1:]) != tuple(ub.shape[1:]):
                 raise SdnsError("%s must be a contiguous (%d,) + %s %s GPU tensor"
                                 % (nm, nc, self.physical_shape, self.real_dtype))
         self._sync_stream()
@@ -258,8 +274,10 @@ class Plan:
         return zz
 
     def _physn(self, t, nc, name):
+        # pointwise kernels: the plan's grid or the padded grid of the 3/2 rule
+        lead = (nc,) if nc > 1 else ()
         if t.device.type != "cuda" or t.dtype != self.real_dtype or not t.is_contiguous() \
-                or tuple(t.shape) != ((nc,) if nc > 1 else ()) + self.physical_shape:
+                or tuple(t.shape) not in (lead + self.physical_shape, lead + self.dealiased_shape):
             raise SdnsError("%s must be a contiguous %s %s GPU tensor of %d component(s)"
                             % (name, self.physical_shape, self.real_dtype, nc))
 
@@ -282,6 +300,8 @@ class Plan:
         self._physn(out, 1, "out")
         self._physn(u, 3, "u")
         self._physn(g, 3, "g")
+        if not (tuple(out.shape[-3:]) == tuple(u.shape[-3:]) == tuple(g.shape[-3:])):
+            raise SdnsError("dot3: grid mismatch")
         self._sync_stream()
         _check(self._lib.sdns_dot3(self._h, _ptr(out), _ptr(u), _ptr(g), out.numel()))
         return out
@@ -290,6 +310,8 @@ class Plan:
         """uu = (u0u0, u0u1, u0u2, u1u1, u1u2, u2u2) on physical arrays."""
         self._physn(uu, 6, "uu")
         self._physn(u, 3, "u")
+        if tuple(uu.shape[-3:]) != tuple(u.shape[-3:]):
+            raise SdnsError("uu_products: grid mismatch")
         self._sync_stream()
         _check(self._lib.sdns_uu_products(self._h, _ptr(uu), _ptr(u), u[0].numel()))
         return uu

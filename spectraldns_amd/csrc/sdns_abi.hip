@@ -320,6 +320,67 @@ template <typename Rt> int plain_forward(sdns_plan* p, const void* up, void* uh,
   return 0;
 }
 
+// 3/2-rule transforms between the N-mode spectral array and the padded
+// physical grid M = 3N/2 (shenfun's padded VTp.backward / Tp.forward, as
+// NS.py:29-32 and MHD.py:30-32 build them): the backward passes read the
+// kept input modes (|k| < N/2 on every axis, the Nyquist modes dropped) into
+// bins of length-M lines whose other bins are zero; the forward passes keep
+// the N-mode band of each padded line (|k| <= N/2 on the R2C axis, -N/2..N/2-1
+// on the others) and scale by 1/(M0 M1 M2).  Intermediates: x done
+// (nc, M0, N1, Nh) in bufA, y done (nc, M0, M1, Nh) in bufB.
+template <typename Rt> int padded_backward(sdns_plan* p, const void* uh, void* up, int nc) {
+  const int N0 = p->N[0], N1 = p->N[1], Nh = p->Nh;
+  const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
+  const long plane = (long)N1 * Nh;
+  StridedArgs<Rt> a{};
+  a.in = uh; a.out = p->bufA; a.tw = p->tw.at(M0);
+  a.Q = (int)plane; a.cdiv = (int)plane; a.cvalid = (int)plane; a.in_hi = 0; a.out_hi = 0;
+  a.in_row = plane; a.out_row = plane; a.in_field = (long)N0 * plane; a.out_field = (long)M0 * plane;
+  a.rin = rmap(p->in[0][0], p->in[0][1], N0);
+  a.rout = rmap(M0 / 2, M0 / 2 - 1, M0);
+  a.scale = Rt(1);
+  if (launch_strided<Rt>(M0, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
+  a.in = p->bufA; a.out = p->bufB; a.tw = p->tw.at(M1);
+  a.Q = M0 * Nh; a.cdiv = Nh; a.cvalid = Nh; a.in_hi = plane; a.out_hi = (long)M1 * Nh; a.in_row = Nh;
+  a.out_row = Nh; a.in_field = (long)M0 * plane; a.out_field = (long)M0 * M1 * Nh;
+  a.rin = rmap(p->in[1][0], p->in[1][1], N1);
+  a.rout = rmap(M1 / 2, M1 / 2 - 1, M1);
+  if (launch_strided<Rt>(M1, p->var[5], +1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  ZArgs<Rt> z{};
+  z.in = p->bufB; z.out = up; z.tw = p->tw.at(M2 / 2); z.rtw = p->rtw.at(M2);
+  z.L = M0 * M1; z.nk = p->in[2][1] + 1; z.in_line = Nh; z.in_field = (long)M0 * M1 * Nh; z.out_line = M2 / 2;
+  z.out_field = (long)M0 * M1 * (M2 / 2); z.scale = Rt(1);
+  if (launch_z<Rt>(M2 / 2, p->var[5], +1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", M2);
+  return 0;
+}
+
+template <typename Rt> int padded_forward(sdns_plan* p, const void* up, void* uh, int nc) {
+  const int N0 = p->N[0], N1 = p->N[1], Nh = p->Nh;
+  const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
+  const long plane = (long)N1 * Nh;
+  ZArgs<Rt> z{};
+  z.in = up; z.out = p->bufA; z.tw = p->tw.at(M2 / 2); z.rtw = p->rtw.at(M2);
+  z.L = M0 * M1; z.nk = p->out[2][1] + 1; z.in_line = M2 / 2; z.in_field = (long)M0 * M1 * (M2 / 2);
+  z.out_line = Nh; z.out_field = (long)M0 * M1 * Nh; z.scale = Rt(1);
+  if (launch_z<Rt>(M2 / 2, p->var[5], -1, nc, z, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "z length %d", M2);
+  StridedArgs<Rt> a{};
+  a.in = p->bufA; a.out = p->bufB; a.tw = p->tw.at(M1);
+  a.Q = M0 * Nh; a.cdiv = Nh; a.cvalid = Nh; a.in_hi = (long)M1 * Nh; a.out_hi = plane; a.in_row = Nh;
+  a.out_row = Nh; a.in_field = (long)M0 * M1 * Nh; a.out_field = (long)M0 * plane;
+  a.rin = rmap(M1 / 2, M1 / 2 - 1, M1);
+  a.rout = rmap(p->out[1][0], p->out[1][1], N1);
+  a.scale = Rt(1);
+  if (launch_strided<Rt>(M1, p->var[5], -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "y length %d", M1);
+  a.in = p->bufB; a.out = uh; a.tw = p->tw.at(M0);
+  a.Q = (int)plane; a.cdiv = (int)plane; a.cvalid = (int)plane; a.in_hi = 0; a.out_hi = 0; a.in_row = plane;
+  a.out_row = plane; a.in_field = (long)M0 * plane; a.out_field = (long)N0 * plane;
+  a.rin = rmap(M0 / 2, M0 / 2 - 1, M0);
+  a.rout = rmap(p->out[0][0], p->out[0][1], N0);
+  a.scale = (Rt)(1.0 / ((double)M0 * M1 * M2));
+  if (launch_strided<Rt>(M0, p->var[5], -1, nc, a, p->stream)) return fail(SDNS_ERR_UNSUPPORTED, "x length %d", M0);
+  return 0;
+}
+
 // Argument blocks of P1..P4 of one RHS evaluation from the state u (device)
 // into W4 (= bufB); shared by the single-box path and the batched ensembles.
 template <typename Rt> P1Args<Rt> p1_args(const sdns_plan* p, const void* u) {
@@ -1021,11 +1082,25 @@ int sdns_backward(sdns_plan* p, const void* uh, void* up, int nc) {
 int sdns_backward_dealiased(sdns_plan* p, const void* uh, void* up, int nc) {
   if (!p || !up || !uh || nc < 1) return fail(SDNS_ERR_INVALID, "bad argument");
   if (nc > 3) return fail(SDNS_ERR_INVALID, "ncomp %d > 3", nc);
-  if (p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
   int rc = ensure_workspace(p);
   if (rc) return rc;
-  rc = p->precision == SDNS_DOUBLE ? plain_backward<double>(p, uh, up, nc, true)
-                                   : plain_backward<float>(p, uh, up, nc, true);
+  if (p->dealias == SDNS_DEALIAS_32)
+    rc = p->precision == SDNS_DOUBLE ? padded_backward<double>(p, uh, up, nc) : padded_backward<float>(p, uh, up, nc);
+  else
+    rc = p->precision == SDNS_DOUBLE ? plain_backward<double>(p, uh, up, nc, true)
+                                     : plain_backward<float>(p, uh, up, nc, true);
+  return rc ? rc : check_launch();
+}
+
+int sdns_forward_dealiased(sdns_plan* p, const void* up, void* uh, int nc) {
+  if (!p || !up || !uh || nc < 1) return fail(SDNS_ERR_INVALID, "bad argument");
+  if (nc > 3) return fail(SDNS_ERR_INVALID, "ncomp %d > 3", nc);
+  int rc = ensure_workspace(p);
+  if (rc) return rc;
+  if (p->dealias == SDNS_DEALIAS_32)
+    rc = p->precision == SDNS_DOUBLE ? padded_forward<double>(p, up, uh, nc) : padded_forward<float>(p, up, uh, nc);
+  else
+    rc = p->precision == SDNS_DOUBLE ? plain_forward<double>(p, up, uh, nc) : plain_forward<float>(p, up, uh, nc);
   return rc ? rc : check_launch();
 }
 
@@ -1069,7 +1144,6 @@ int sdns_vv_rhs_tail(sdns_plan* p, void* rhs, const void* v_hat, const void* w_h
 }
 int sdns_dealias_truncate(sdns_plan* p, void* out, const void* a) {
   if (!a) return fail(SDNS_ERR_INVALID, "null input");
-  if (p && p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
   return ew(p, EW_TRUNCATE, out, a, nullptr, nullptr, 0, -1);
 }
 int sdns_set_elsasser(sdns_plan* p, void* c_hat, const void* zz_hat) {
@@ -1078,7 +1152,6 @@ int sdns_set_elsasser(sdns_plan* p, void* c_hat, const void* zz_hat) {
 }
 int sdns_gradient_truncated(sdns_plan* p, void* out, const void* a) {
   if (!a) return fail(SDNS_ERR_INVALID, "null input");
-  if (p && p->dealias == SDNS_DEALIAS_32) return fail(SDNS_ERR_UNSUPPORTED, "the 3/2 rule pads instead");
   return ew(p, EW_GRAD_TRUNC, out, a, nullptr, nullptr, 0, -1);
 }
 int sdns_dot3(sdns_plan* p, void* out, const void* u, const void* g, size_t npoints) {

@@ -33,7 +33,8 @@ class NyquistMask:
 class FourierSpace:
     """rank 0 (scalar) or rank 1 (3-vector) Fourier space of one box.
     ``dealiased`` marks the padded / truncated space of the nonlinear term
-    (NS.py:29-32); its transforms run fused inside the RHS kernels."""
+    (NS.py:29-32): Tp / VTp, whose transforms are sdns_backward_dealiased and
+    sdns_forward_dealiased (the Vortex RHS runs them fused in its passes)."""
 
     def __init__(self, plan, rank=0, dealiased=False):
         self.plan = plan
@@ -44,8 +45,8 @@ class FourierSpace:
     def shape(self, forward_output=False):
         if forward_output:
             s = self.plan.spectral_shape
-        elif self.dealiased and self.plan.dealias == "3/2-rule":
-            s = tuple(3 * n // 2 for n in self.plan.N)
+        elif self.dealiased:
+            s = self.plan.dealiased_shape
         else:
             s = self.plan.N
         return ((3,) if self.rank else ()) + tuple(s)
@@ -55,25 +56,25 @@ class FourierSpace:
         return 3
 
     # -- transforms -----------------------------------------------------------
-    def _plain(self):
-        if self.dealiased:
-            raise SdnsError("transforms of the dealiased space run fused inside ComputeRHS; "
-                            "use the unpadded space for standalone transforms")
-
+    # The dealiased space (Tp, VTp) transforms between the plan's modes and its
+    # dealiased grid: the padded 3N/2 grid of the 3/2 rule, or the plan's grid
+    # with the 2/3-rule input truncation (backward) and a plain forward.
     def forward(self, u, u_hat=None):
-        self._plain()
         import torch
         if u_hat is None:
             u_hat = torch.empty(u.shape[:-3] + self.plan.spectral_shape, dtype=self.plan.complex_dtype,
                                 device=u.device)
+        if self.dealiased:
+            return self.plan.forward_dealiased(u, u_hat)
         return self.plan.forward(u, u_hat)
 
     def backward(self, u_hat, u=None):
-        self._plain()
         import torch
         if u is None:
-            u = torch.empty(u_hat.shape[:-3] + self.plan.physical_shape, dtype=self.plan.real_dtype,
-                            device=u_hat.device)
+            shp = self.plan.dealiased_shape if self.dealiased else self.plan.physical_shape
+            u = torch.empty(u_hat.shape[:-3] + shp, dtype=self.plan.real_dtype, device=u_hat.device)
+        if self.dealiased:
+            return self.plan.backward_dealiased(u_hat, u)
         return self.plan.backward(u_hat, u)
 
     # -- mesh / wavenumbers ---------------------------------------------------

@@ -13,9 +13,10 @@ products (u_i + b_i)(u_j - b_j), their forward transforms, one kernel for
 set_Elsasser, the Nyquist mask, the NS pressure/diffusion kernel on U and
 the diffusion kernel on B.
 
-Built for the 2/3 rule (shenfun's dealias_direct, dealias_mode 0) and for no
-dealiasing; the 3/2 rule would need padded transforms of six fields and
-raises.
+Built for the 2/3 rule (shenfun's dealias_direct, dealias_mode 0), the 3/2
+rule (the six fields transformed onto the padded 3N/2 grid, the nine
+products transformed back and truncated, MHD.py:30-32) and for no
+dealiasing.
 """
 from .spectralinit import *  # noqa: F401,F403
 from .spectralinit import config, params, datatypes, HDF5File  # noqa: F401
@@ -34,11 +35,16 @@ class CompositeSpace:
         self.plan, self.ncomp, self.dealiased = plan, ncomp, dealiased
 
     def shape(self, forward_output=False):
-        return (self.ncomp,) + tuple(self.plan.spectral_shape if forward_output else self.plan.N)
+        if forward_output:
+            return (self.ncomp,) + tuple(self.plan.spectral_shape)
+        return (self.ncomp,) + tuple(self.plan.dealiased_shape if self.dealiased else self.plan.N)
 
     def forward(self, u, u_hat):
         for c in range(0, u.shape[0], 3):
-            self.plan.forward(u[c:c + 3], u_hat[c:c + 3])
+            if self.dealiased:
+                self.plan.forward_dealiased(u[c:c + 3], u_hat[c:c + 3])
+            else:
+                self.plan.forward(u[c:c + 3], u_hat[c:c + 3])
         return u_hat
 
     def backward(self, u_hat, u):
@@ -54,8 +60,6 @@ def get_context():
     """Context of the MHD solver (MHD.py:12-79)."""
     import torch
     float, complex, mpitype = datatypes(params.precision)
-    if params.dealias == "3/2-rule":
-        raise NotImplementedError("MHD with the 3/2 rule (padded transforms of six fields) is not built")
     if int(params.get("dealias_mode", 0)) != 0:
         raise NotImplementedError("MHD runs shenfun's 2/3 rule (dealias_mode 0)")
     plan = Plan(params.N, params.L, params.precision, params.dealias, params.mask_nyquist, 0)
@@ -120,11 +124,11 @@ def divergenceConvection(c, ub_dealias, Tp, K, ZZ_hat):
     plan = K.plan
     zz = getattr(Tp, "_zz", None)
     if zz is None:
-        zz = Tp._zz = torch.empty((9,) + tuple(plan.physical_shape), dtype=plan.real_dtype, device=plan.device)
+        zz = Tp._zz = torch.empty((9,) + tuple(plan.dealiased_shape), dtype=plan.real_dtype, device=plan.device)
     plan.elsasser_products(zz, ub_dealias)
     flat = ZZ_hat.view((9,) + tuple(ZZ_hat.shape[2:]))
     for k in range(0, 9, 3):
-        plan.forward(zz[k:k + 3], flat[k:k + 3])
+        plan.forward_dealiased(zz[k:k + 3], flat[k:k + 3])
     return set_Elsasser(c, ZZ_hat, K)
 
 

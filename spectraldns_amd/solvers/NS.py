@@ -132,24 +132,25 @@ def Cross(c, a, b, work, T):
 
 
 def _dealiased_backward(plan, u_hat, u):
-    """VTp.backward(u_hat) for the 2/3 rule or no dealiasing: the inverse
-    transform of the kept modes (truncation inside the transform passes)."""
+    """VTp.backward(u_hat): the inverse transform of the kept modes (2/3 rule or
+    no dealiasing, truncation inside the transform passes) or onto the padded
+    3N/2 grid (3/2 rule)."""
     return plan.backward_dealiased(u_hat, u)
 
 
 def standard_convection(rhs, u_dealias, U_hat, work, Tp, K):
-    """rhs_i = F(u_j du_i/dx_j) (NS.py:138-145): per component, the truncated
-    gradient 1j*K_j*U_hat_i, its inverse transform, the dot product with
-    u_dealias and the forward transform."""
+    """rhs_i = Tp.forward(u_j du_i/dx_j) (NS.py:138-145): per component, the
+    truncated gradient 1j*K_j*U_hat_i, its dealiased inverse transform, the dot
+    product with u_dealias and the dealiased forward transform."""
     plan = K.plan
     gradUi = work[(u_dealias, 1, False)]
     g_hat = work[(U_hat, 2, False)]
     s = work[(u_dealias[0], 0, False)]
     for i in range(3):
         plan.gradient_truncated(g_hat, U_hat[i])
-        plan.backward(g_hat, gradUi)
+        plan.backward_dealiased(g_hat, gradUi)
         plan.dot3(s, u_dealias, gradUi)
-        plan.forward(s, rhs[i])
+        plan.forward_dealiased(s, rhs[i])
     return rhs
 
 
@@ -161,12 +162,12 @@ def divergence_convection(rhs, u_dealias, work, Tp, K, add=False, scale=1.0):
     bufs = getattr(plan, "_uu_buffers", None)
     if bufs is None:
         bufs = plan._uu_buffers = (
-            torch.empty((6,) + tuple(plan.physical_shape), dtype=plan.real_dtype, device=plan.device),
+            torch.empty((6,) + tuple(plan.dealiased_shape), dtype=plan.real_dtype, device=plan.device),
             torch.empty((6,) + tuple(plan.spectral_shape), dtype=plan.complex_dtype, device=plan.device))
     uu, uu_hat = bufs
     plan.uu_products(uu, u_dealias)
-    plan.forward(uu[0:3], uu_hat[0:3])
-    plan.forward(uu[3:6], uu_hat[3:6])
+    plan.forward_dealiased(uu[0:3], uu_hat[0:3])
+    plan.forward_dealiased(uu[3:6], uu_hat[3:6])
     if add:
         d = work[(rhs, 3, False)]
         plan.div_uu(d, uu_hat, scale)
@@ -181,9 +182,9 @@ def getConvection(convection):
 
     The Vortex form F(u x curl u) on the dealiased space is one fused launch
     sequence (2/3 and 3/2 rule).  The Standard, Divergence and Skewed forms
-    run their transforms and products as separate launches, for the 2/3 rule
-    and without dealiasing (the 3/2 rule raises: its padded transforms exist
-    only inside the fused passes)."""
+    run their transforms and products as separate launches: the dealiased
+    transforms of the plan (padded 3N/2 grid for the 3/2 rule, input
+    truncation for the 2/3 rule) and the pointwise product kernels."""
     if convection == "Vortex":
         def Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias=None):
             return VTp.plan.convection(u_hat, rhs)
@@ -196,8 +197,6 @@ def getConvection(convection):
 
     def Conv(rhs, u_hat, work, Tp, VTp, K, u_dealias):
         plan = VTp.plan
-        if plan.dealias == "3/2-rule":
-            raise NotImplementedError("the %s form is built for the 2/3 rule and no dealiasing" % convection)
         u_dealias = _dealiased_backward(plan, u_hat, u_dealias)
         if convection == "Standard":
             rhs = standard_convection(rhs, u_dealias, u_hat, work, Tp, K)

@@ -1,7 +1,7 @@
 """MHD in Elsasser form (spectralDNS/solvers/MHD.py) on the GPU, driven through
 the reference's API as tests/test_MHD.py:32-49 drives it (Taylor-Green U and
 B of tests/TGMHD.py:4-14 on the uniform and the non-uniform box, nu =
-0.000625, eta = 0.01, Divergence convection, RK4, 2/3 rule; pins k(0.1) =
+0.000625, eta = 0.01, Divergence convection, RK4, 2/3 rule and 3/2 rule; pins k(0.1) =
 0.124565408177 and b(0.1) = 0.124637762143 at ntol 7), and compared with the
 MHD oracle (oracle/mhd.py): FP64 within 1e-12 relative L2 after the ten
 steps.  The three MHD kernels against their NumPy definitions."""
@@ -25,10 +25,11 @@ CASES = [["--M", "4", "4", "4", "--L", "2*pi", "2*pi", "2*pi"],
          ["--M", "6", "5", "4", "--L", "6*pi", "4*pi", "2*pi"]]
 
 
+@pytest.mark.parametrize("rule", ["2/3-rule", "3/2-rule"])
 @pytest.mark.parametrize("args", CASES)
-def test_mhd_taylor_green_fp64(args):
+def test_mhd_taylor_green_fp64(args, rule):
     from spectraldns_amd import config, get_solver, solve
-    print("\n[case] MHD %s" % " ".join(args), flush=True)
+    print("\n[case] MHD %s %s" % (" ".join(args), rule), flush=True)
     seen = {}
 
     def regression(context):
@@ -40,7 +41,7 @@ def test_mhd_taylor_green_fp64(args):
         assert round(seen["b"] - 0.124637762143, 7) == 0, seen
 
     config.update({"nu": 0.000625, "dt": 0.01, "T": 0.1, "eta": 0.01, "convection": "Divergence",
-                   "dealias": "2/3-rule", "precision": "double"})
+                   "dealias": rule, "precision": "double"})
     solver = get_solver(regression_test=regression, parse_args=args + ["MHD"])
     c = solver.get_context()
     X = c.X
@@ -56,10 +57,10 @@ def test_mhd_taylor_green_fp64(args):
     solve(solver, c)
     ub = c.UB_hat.cpu().numpy()
     div = float(solver.get_divergence(**c).abs().max())
-    config.update({"convection": "Vortex"})
+    config.update({"convection": "Vortex", "dealias": "2/3-rule"})
     M = [int(a) for a in args[1:4]]
     L = (TWO_PI * 3, TWO_PI * 2, TWO_PI) if M[0] == 6 else (TWO_PI,) * 3
-    g = Grid(tuple(2 ** m for m in M), L, "double", "2/3-rule")
+    g = Grid(tuple(2 ** m for m in M), L, "double", rule)
     ref = OM.taylor_green(g)
     for dt in dt_schedule(0.1, 0.01):
         ref = OM.rk4_step(g, ref, dt, 0.000625, 0.01)

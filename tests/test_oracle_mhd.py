@@ -1,7 +1,8 @@
 """Pins of the MHD oracle (oracle/mhd.py), on the CPU: the reference's
 regression values k(0.1), b(0.1) of tests/TGMHD.py:16-25 at 7 decimals on the
 two boxes of tests/test_MHD.py:15-30 (2/3 rule, nu = 0.000625, eta = 0.01),
-and the b = 0 reduction to the NS oracle on the modes the 2/3 rule keeps."""
+the same pins with the 3/2 rule, and the b = 0 reduction to the NS oracle on
+the modes the 2/3 rule keeps and, with the 3/2 rule, on every mode."""
 import numpy as np
 import pytest
 
@@ -11,9 +12,10 @@ from oracle.ns import Grid, dt_schedule
 BOXES = [((4, 4, 4), (2 * np.pi,) * 3), ((6, 5, 4), (6 * np.pi, 4 * np.pi, 2 * np.pi))]
 
 
+@pytest.mark.parametrize("rule", ["2/3-rule", "3/2-rule"])
 @pytest.mark.parametrize("M,L", BOXES)
-def test_mhd_taylor_green_pins(M, L):
-    g = Grid(tuple(2 ** m for m in M), L, "double", "2/3-rule")
+def test_mhd_taylor_green_pins(M, L, rule):
+    g = Grid(tuple(2 ** m for m in M), L, "double", rule)
     ub = OM.taylor_green(g)
     for dt in dt_schedule(0.1, 0.01):
         ub = OM.rk4_step(g, ub, dt, 0.000625, 0.01)
@@ -39,3 +41,20 @@ def test_mhd_without_field_is_ns():
     err = np.linalg.norm((ub[:3] - u)[:, keep]) / np.linalg.norm(u[:, keep])
     assert err < 1e-12, err
     assert np.max(np.abs(ub[3:])) == 0
+
+
+def test_mhd_without_field_is_ns_32_rule():
+    """With padding the Elsasser products u_i u_j are exact on the kept modes, so
+    the divergence form equals the rotational one up to the gradient that the
+    projection removes: every mode follows the NS oracle."""
+    from spectraldns_amd.isotropic import random_solenoidal_field
+    N = (16, 16, 16)
+    g = Grid(N, (2 * np.pi,) * 3, "double", "3/2-rule")
+    u = random_solenoidal_field(N, seed=5, k0=3.0, energy=0.5)
+    ub = np.concatenate([u, np.zeros_like(u)])
+    for _ in range(2):
+        u = g.rk4_step(u, 0.01, 0.01)
+        ub = OM.rk4_step(g, ub, 0.01, 0.01, 0.02)
+    err = np.linalg.norm((ub[:3] - u).ravel()) / np.linalg.norm(u.ravel())
+    assert err < 1e-13, err
+    assert np.abs(ub[3:]).max() == 0.0

@@ -281,6 +281,39 @@ __device__ __forceinline__ void c2r_pre_lds(cx<Rt> (&z)[E], const cx<Rt>* my, co
   }
 }
 
+// c2r pre-processing of a staged line: the thread's own bins X[p], p = t + P e,
+// come from its staging registers r[e]; only the partners X[n - p] are read
+// from the line's LDS area.  Bins at or above NKI are zero on every line (the
+// dealiased input keeps fewer bins), so elements whose own bin or partner is
+// at or above NKI for every thread are compile-time zeros.
+template <typename Rt, int n, int E, int NKI, int K>
+__device__ __forceinline__ void c2r_pre_reg(cx<Rt> (&z)[E], const cx<Rt> (&r)[K], const cx<Rt>* my,
+                                            const cx<Rt>* __restrict__ rtw, int t) {
+  using C = cx<Rt>;
+  constexpr int P = n / E;
+  static_assert(K >= E, "staging registers hold the thread's E bins");
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + P * e;
+    C xa = (P * e >= NKI) ? czero<C>() : r[e];
+    C xb = (n - (P * e + P - 1) >= NKI) ? czero<C>() : my[n - p];
+    if (p == 0) { xa.y = 0; xb.y = 0; }
+    const C ev = mk<C>(xa.x + xb.x, xa.y - xb.y);
+    const C d = mk<C>(xa.x - xb.x, xa.y + xb.y);
+    const C od = cmulc(d, rtw[p]);
+    z[e] = mk<C>(ev.x - od.y, ev.y + od.x);
+  }
+}
+// Number of staged bins per thread (t + P k, k < this) that c2r_pre_reg reads
+// as partners: up to index n - P e_lo, e_lo the first element with a partner
+// below NKI.
+constexpr int stage_kept(int n, int P, int NKI, int K) {
+  int elo = 0;
+  while (n - (P * elo + P - 1) >= NKI) ++elo;
+  const int k = (n - P * elo) / P + 1;
+  return k < K ? k : K;
+}
+
 // Bin k of the real transform from the half-length FFT values A = Z[k] and
 // Bz = Z[(n - k) mod n], w = rtw[k].
 template <typename C> __device__ __forceinline__ C r2c_bin(C A, C Bz, C w) {
@@ -315,16 +348,22 @@ __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __r
 // r2c_post writing through a buffer descriptor: `base` is the byte offset of
 // this line in the descriptor's span (lines outside the span are dropped by
 // the range check); unit scale.
-template <typename Rt, int n, int E, bool WAVE = false>
+// NKO: output bins at or above NKO are never needed (compile-time bound of nk),
+// so their elements are skipped and only the partners of the needed ones are
+// written to LDS.
+template <typename Rt, int n, int E, bool WAVE = false, int NKO = n + 1>
 __device__ __forceinline__ void r2c_post_buf(cx<Rt> (&z)[E], cx<Rt>* my, __amdgpu_buffer_rsrc_t rs, int base, int nk,
                                              const cx<Rt>* __restrict__ rtw, int t) {
   using C = cx<Rt>;
   constexpr int P = n / E, ES = (int)sizeof(C);
+  constexpr int EHI = (NKO + P - 1) / P < E ? (NKO + P - 1) / P : E;  // elements with a needed bin
+  constexpr int PLO = n - P * EHI + 1;                                 // lowest partner index read (k > 0)
 #pragma unroll
-  for (int e = 0; e < E; ++e) my[t + P * e] = z[e];
+  for (int e = 0; e < E; ++e)
+    if (e == 0 || P * e + P - 1 >= PLO) my[t + P * e] = z[e];
   lds_barrier<WAVE>();
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
+  for (int e = 0; e < EHI; ++e) {
     const int k = t + P * e;
     const C X = r2c_bin(z[e], my[(n - k) % n], rtw[k]);
     buf_store<C>(X, rs, t < nk - P * e ? base + ES * k : BUF_OOB);
@@ -355,9 +394,11 @@ struct LineStage {
 #pragma unroll
     for (int k = 0; k < K; ++k) r[k] = buf_load<cx<Rt>>(rs, t < nk - P * k ? base + ES * (t + P * k) : BUF_OOB);
   }
-  __device__ __forceinline__ void store(cx<Rt>* my, int t) const {
+  __device__ __forceinline__ void store(cx<Rt>* my, int t) const { store_first<K>(my, t); }
+  // the first KS staged bins only (the partners that c2r_pre_reg reads)
+  template <int KS> __device__ __forceinline__ void store_first(cx<Rt>* my, int t) const {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
+    for (int k = 0; k < KS; ++k) {
       const int i = t + P * k;
       if (i <= n) my[i] = r[k];
     }
@@ -432,11 +473,14 @@ template <typename Rt, int n, int E_> struct ZBCfg {
   static constexpr int THREADS = LPB * P;
 };
 
-template <typename Rt, int n, int E, int LPB>
+// NKI / NKO: compile-time bounds of the input bins held (a.nkz <= NKI) and of
+// the output bins needed (a.nkzo <= NKO); n + 1 for the generic pass.
+template <typename Rt, int n, int E, int LPB, int NKI = n + 1, int NKO = n + 1>
 __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>* lds, const cx<Rt>* tw,
                                          const cx<Rt>* rtw) {
   using C = cx<Rt>;
   constexpr int P = n / E, SL = z3_slot(n, P, (int)sizeof(cx<Rt>));
+  constexpr int KS = stage_kept(n, P, NKI, LineStage<Rt, n, P>::K);
   constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
@@ -458,9 +502,9 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   // velocity lines -> registers (staging and exchange in slot 0)
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    st.store(s0, t);
+    st.template store_first<KS>(s0, t);
     lds_barrier<WL>();
-    c2r_pre_lds<Rt, n, E>(U[f], s0, rtw, t);
+    c2r_pre_reg<Rt, n, E, NKI>(U[f], st.r, s0, rtw, t);
     lds_barrier<WL>();
     st.load_buf(buf_rsrc(inb + (f + 1) * ifield, ibytes), ibase, a.nkz, t);
     Fi::run(U[f], t, s0, 1, tw);
@@ -469,9 +513,9 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
     C* sl = f == 0 ? s1 : s2;
-    st.store(sl, t);
+    st.template store_first<KS>(sl, t);
     lds_barrier<WL>();
-    c2r_pre_lds<Rt, n, E>(W, sl, rtw, t);
+    c2r_pre_reg<Rt, n, E, NKI>(W, st.r, sl, rtw, t);
     lds_barrier<WL>();
     st.load_buf(buf_rsrc(inb + (4 + f) * ifield, ibytes), ibase, a.nkz, t);
     Fi::run(W, t, sl, 1, tw);
@@ -479,9 +523,9 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
     for (int e = 0; e < E; ++e) sl[t + P * e] = W[e];
   }
   // w2 -> registers (slot 0)
-  st.store(s0, t);
+  st.template store_first<KS>(s0, t);
   lds_barrier<WL>();
-  c2r_pre_lds<Rt, n, E>(W, s0, rtw, t);
+  c2r_pre_reg<Rt, n, E, NKI>(W, st.r, s0, rtw, t);
   lds_barrier<WL>();
   Fi::run(W, t, s0, 1, tw);
   lds_barrier<WL>();
@@ -504,11 +548,11 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   for (int i = 0; i < 3; ++i) {
     C* sl = i == 0 ? s0 : (i == 1 ? s1 : s2);
     Ff::run(U[i], t, sl, 1, tw);
-    r2c_post_buf<Rt, n, E, WL>(U[i], sl, buf_rsrc(outb + i * ofield, obytes), obase, a.nkzo, rtw, t);
+    r2c_post_buf<Rt, n, E, WL, NKO>(U[i], sl, buf_rsrc(outb + i * ofield, obytes), obase, a.nkzo, rtw, t);
   }
 }
 
-template <typename Rt, int n, int E>
+template <typename Rt, int n, int E, int NKI = n + 1, int NKO = n + 1>
 __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3b(P3Args<Rt> a) {
   using C = cx<Rt>;
   constexpr int LPB = ZBCfg<Rt, n, E>::LPB;
@@ -516,7 +560,7 @@ __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + s
   __shared__ C tw[n], rtw[n];
   table_to_lds(tw, (const C*)a.tw, n);
   table_to_lds(rtw, (const C*)a.rtw, n);
-  p3b_body<Rt, n, E, LPB>(a, blockIdx.x * LPB, lds, tw, rtw);
+  p3b_body<Rt, n, E, LPB, NKI, NKO>(a, blockIdx.x * LPB, lds, tw, rtw);
 }
 
 // ------------------------------------------------------------------ z pass, LDS-DMA scheme

@@ -21,9 +21,24 @@ template <typename Rt, int n, int E> void p3_go(const P3Args<Rt>& a, hipStream_t
   using Cfg = ZCfg<Rt, n, E>;
   hipLaunchKernelGGL((k_p3<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
 }
+// The LDS-assisted z pass, specialised (for n >= 64) on the dealiased bin
+// counts: the 3/2 rule holds 2n/3 input bins and needs 2n/3 + 1 outputs, the
+// 2/3 rule holds ceil(2n/3) inputs; any other plan gets the generic kernel.
 template <typename Rt, int n, int E> void p3b_go(const P3Args<Rt>& a, hipStream_t s) {
   using Cfg = ZBCfg<Rt, n, E>;
-  hipLaunchKernelGGL((k_p3b<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
+  const dim3 g((a.L + Cfg::LPB - 1) / Cfg::LPB), b(Cfg::THREADS);
+  if constexpr (n >= 64) {
+    constexpr int K32 = 2 * n / 3, K23 = (2 * n + 2) / 3;
+    if (n % 3 == 0 && a.nkz <= K32 && a.nkzo <= K32 + 1) {
+      hipLaunchKernelGGL((k_p3b<Rt, n, E, K32, K32 + 1>), g, b, 0, s, a);
+      return;
+    }
+    if (a.nkz <= K23) {
+      hipLaunchKernelGGL((k_p3b<Rt, n, E, K23, n + 1>), g, b, 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_p3b<Rt, n, E>), g, b, 0, s, a);
 }
 // Scheme C (LDS-DMA staging) with LPB lines per block, if its block is whole
 // waves and its LDS fits in 64 KB; false otherwise.

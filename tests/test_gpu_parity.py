@@ -226,8 +226,9 @@ def test_alternative_schedules_fp64(N, rule):
                                     ((32, 32, 256), "3/2-rule"), ((64, 64, 96), "None")])
 def test_z_pass_variants_fp64(N, rule):
     """Every storage scheme / schedule of the z pass (variants 0..5) keeps FP64
-    parity with the oracle; variants 4/5 (rows staged by direct HBM -> LDS
-    loads) equal variants 2/3 bit for bit (same arithmetic)."""
+    parity with the oracle (variants 2/3 skip the bins that dealiasing zeroes,
+    so their sums are grouped differently from variants 4/5 and agree with
+    them to rounding, not bit for bit)."""
     L = (TWO_PI, 4.0, 3.0)
     nu = 0.01
     g = Grid(N, L, "double", rule)
@@ -242,7 +243,7 @@ def test_z_pass_variants_fp64(N, rule):
         p.compute_rhs(ud, rhs, nu)
         out[var] = rhs.cpu().numpy()
         assert rel(out[var], ref) < 1e-13, var
-    assert np.array_equal(out[2], out[4]) and np.array_equal(out[3], out[5])
+    assert rel(out[2], out[4]) < 1e-14 and rel(out[3], out[5]) < 1e-14
 
 
 @pytest.mark.parametrize("N", [(1024, 32, 64), (32, 64, 1024)])

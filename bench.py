@@ -396,6 +396,15 @@ def main():
         print("bench.py measures one GPU: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * 2 ** 20 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "ensemble":
         ensemble_main(args)
         return
@@ -406,15 +415,6 @@ def main():
         solver_main(args)
         return
     N = (args.N,) * 3
-    if args.calibrate_copy:
-        import torch
-        n = args.calibrate_copy * 2 ** 20 // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     elapsed, passes, diag, variants, dom = gpu_run(N, "single", args.dealias, args.steps, args.warmup)
     ms_step = 1000.0 * elapsed / args.steps
     sps = args.steps / elapsed

@@ -284,6 +284,16 @@ int sdns_batch_set_stream(sdns_batch* batch, void* hip_stream);
  * 2*nbox doubles receiving (kinetic energy, dissipation) of every box after
  * the last step, in FP64, with the definitions of sdns_rk4_steps. */
 int sdns_batch_rk4_steps(sdns_batch* batch, int nsteps, double* diag);
+/* Schedule of one batched pass for one line length (no counterpart in the
+ * reference, which has no batched path): pass 0..4 (x-inverse, y-inverse +
+ * curl, z pass, y-forward, x-forward + RK), length = the line length of that
+ * pass (3N/2 under the 3/2 rule; N2/2 or 3N2/4 for the z pass), variant 0
+ * (default) or 1 (half the points per thread: twice the threads per line and
+ * half the columns or lines per block, where that line plan exists; otherwise
+ * the same as 0).  Lays the work lists out again and drops the captured step;
+ * the results do not depend on the variant beyond rounding.  Set by the
+ * ensemble autotuner (spectraldns_amd/autotune.py). */
+int sdns_batch_set_variant(sdns_batch* batch, int pass, int length, int variant);
 /* The single-box plan of box i (owned by the batch; for diagnostics such as
  * sdns_energy or sdns_spectrum on that box). */
 int sdns_batch_box_plan(sdns_batch* batch, int i, sdns_plan** plan);

@@ -72,6 +72,7 @@ _SIGS = {
     "sdns_batch_set_stream": (ctypes.c_int, [_P, _P]),
     "sdns_batch_rk4_steps": (ctypes.c_int, [_P, ctypes.c_int, _P]),
     "sdns_batch_box_plan": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P)]),
+    "sdns_batch_set_variant": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "sdns_closure_nparams": (ctypes.c_size_t, []),
     "sdns_closure_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32), _P]),
     "sdns_closure_destroy": (ctypes.c_int, [_P]),
@@ -572,6 +573,7 @@ class Batch:
             raise SdnsError("an ensemble needs at least one box")
         self.nbox = len(self.boxes)
         self.precision = precision
+        self.dealias = dealias
         self.complex_dtype = torch.complex64 if precision == "single" else torch.complex128
         self.device = torch.device("cuda", torch.cuda.current_device())
         Na = (ctypes.c_int32 * (3 * self.nbox))(*[n for N, _ in self.boxes for n in N])
@@ -616,6 +618,21 @@ class Batch:
 
     def use_graph(self, on=True):
         _check(self._lib.sdns_batch_use_graph(self._h, int(bool(on))))
+
+    def line_lengths(self):
+        """{pass id: sorted line lengths of that batched pass over the boxes}:
+        x and y lines of the (padded) grid, half z lines for the z pass."""
+        pad = (lambda n: 3 * n // 2) if self.dealias == "3/2-rule" else (lambda n: n)
+        out = {}
+        for N, _ in self.boxes:
+            M = [pad(n) for n in N]
+            for k, l in enumerate((M[0], M[1], M[2] // 2, M[1], M[0])):
+                out.setdefault(k, set()).add(l)
+        return {k: sorted(v) for k, v in out.items()}
+
+    def set_variant(self, pass_id, length, variant):
+        """Schedule variant (0 or 1) of one batched pass for one line length."""
+        _check(self._lib.sdns_batch_set_variant(self._h, int(pass_id), int(length), int(variant)))
 
     def rk4_steps(self, nsteps, diag=None):
         """nsteps RK4 steps of every box; diag (float64 GPU tensor of 2*nbox

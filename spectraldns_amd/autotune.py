@@ -10,6 +10,14 @@ result is stored per box in ``schedules.json`` next to this file and applied
 by ``apply_schedule`` whenever a plan for that box is created.
 
     python -m spectraldns_amd.autotune --N 256 256 256 --precision single --dealias 3/2-rule
+
+Batched ensembles (``_abi.Batch``) have a schedule per (pass, line length):
+variant 0 or 1 (half the points per thread, twice the threads per line).
+``tune_batch`` picks them by timing whole ensemble steps (graph replay, the
+production path), one (pass, length) choice at a time, and stores them under
+the ensemble's size list; ``Ensemble`` applies them on creation.
+
+    python -m spectraldns_amd.autotune --ensemble --precision single --dealias 2/3-rule
 """
 import argparse
 import json
@@ -43,8 +51,8 @@ def apply_schedule(plan, path=SCHEDULE_FILE):
 
 
 def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, steps=3, seed=1234):
-    """Time both variants of every pass on RK4 steps of a random solenoidal
-    field; return {"variants": {pass: v}, "ms": {pass: [ms_v0, ms_v1]}}."""
+    """Time every variant of every pass on RK4 steps of a random solenoidal
+    field; return {"variants": {pass: v}, "ms": {pass: [ms_v0, ms_v1, ...]}}."""
     import torch
     from ._abi import Plan
     from .isotropic import random_solenoidal_field
@@ -73,6 +81,64 @@ def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, 
     return res
 
 
+def batch_key(sizes, precision, dealias, mode=0):
+    """Key of an ensemble: its box sizes with multiplicities, e.g. 32x32,48x32."""
+    counts = {}
+    for n in sizes:
+        counts[int(n)] = counts.get(int(n), 0) + 1
+    desc = ",".join("%dx%d" % (n, c) for n, c in sorted(counts.items()))
+    return "batch/%s/%s/%s/%d" % (desc, precision, dealias, int(mode))
+
+
+def apply_batch_schedule(batch, sizes, precision, dealias, mode=0, path=SCHEDULE_FILE):
+    """Set the stored (pass, length) variants of this ensemble (no-op if none)."""
+    entry = load_table(path).get(batch_key(sizes, precision, dealias, mode))
+    if not entry:
+        return None
+    for item in entry.get("variants", []):
+        batch.set_variant(int(item[0]), int(item[1]), int(item[2]))
+    return entry
+
+
+def _time_ensemble(ens, steps, reps=3):
+    import torch
+    ens.steps(1)                         # captures the step graph for the current layout
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ens.steps(steps)
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b) / steps)
+    return float(np.median(times))
+
+
+def tune_batch(sizes=None, precision="single", dealias="2/3-rule", mode=0, steps=10):
+    """One sweep over the (pass, length) schedule choices of an ensemble:
+    keep variant 1 where it makes the ensemble step faster.  Returns
+    {"variants": [[pass, length, v], ...], "ms": {"pass/length": [ms_v0, ms_v1]},
+    "step_ms": [before, after]}."""
+    from .ensemble import Ensemble
+    ens = Ensemble(sizes, precision=precision, dealias=dealias, dealias_mode=mode, tuned=False)
+    base = _time_ensemble(ens, steps)
+    start = base
+    chosen, ms = [], {}
+    for k, lengths in sorted(ens.batch.line_lengths().items()):
+        for length in lengths:
+            ens.reset()
+            ens.batch.set_variant(k, length, 1)
+            t1 = _time_ensemble(ens, steps)
+            ms["%d/%d" % (k, length)] = [round(base, 5), round(t1, 5)]
+            if t1 < base:
+                base = t1
+                chosen.append([k, length, 1])
+            else:
+                ens.batch.set_variant(k, length, 0)
+    return {"variants": chosen, "ms": ms, "step_ms": [round(start, 5), round(base, 5)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--N", type=int, nargs=3, default=[256, 256, 256])
@@ -81,13 +147,21 @@ def main():
     ap.add_argument("--mode", type=int, default=0)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--out", default=SCHEDULE_FILE)
+    ap.add_argument("--ensemble", action="store_true", help="tune the default 96-box ensemble instead")
     args = ap.parse_args()
-    res = tune(tuple(args.N), precision=args.precision, dealias=args.dealias, mode=args.mode, steps=args.steps)
+    if args.ensemble:
+        from .ensemble import default_sizes
+        sizes = default_sizes()
+        res = tune_batch(sizes, precision=args.precision, dealias=args.dealias, mode=args.mode)
+        key = batch_key(sizes, args.precision, args.dealias, args.mode)
+    else:
+        res = tune(tuple(args.N), precision=args.precision, dealias=args.dealias, mode=args.mode, steps=args.steps)
+        key = key_of(args.N, args.precision, args.dealias, args.mode)
     table = load_table(args.out)
-    table[key_of(args.N, args.precision, args.dealias, args.mode)] = res
+    table[key] = res
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    print(json.dumps({key_of(args.N, args.precision, args.dealias, args.mode): res}), flush=True)
+    print(json.dumps({key: res}), flush=True)
 
 
 if __name__ == "__main__":

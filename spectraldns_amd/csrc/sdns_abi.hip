@@ -751,6 +751,7 @@ struct sdns_batch {
   bool have_state = false, have_params = false, dirty = true, use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::map<std::pair<int, int>, int> sched;  // (pass, line length) -> schedule variant
 };
 
 namespace {
@@ -785,20 +786,29 @@ std::vector<BatchWork> batch_work(const std::vector<int>& ntile, const std::vect
 bool batch_strided_ok(int M) { return batch_E(M) > 0; }
 bool batch_half_ok(int n) { return batch_zE(n) > 0; }
 
+// Schedule variant of pass k for line length l: the one set by
+// sdns_batch_set_variant, or 0; a variant whose plan equals variant 0's is 0.
+int batch_variant(const sdns_batch* b, int k, int l) {
+  const auto it = b->sched.find({k, l});
+  if (it == b->sched.end() || it->second == 0) return 0;
+  return (k == 2 ? batch_zE(l, 1) != batch_zE(l, 0) : batch_E(l, 1) != batch_E(l, 0)) ? 1 : 0;
+}
+
 template <typename Rt> int batch_layout(sdns_batch* b) {
   const int nb = b->nbox;
   std::vector<int> len[5], nt[5], wt[5];
+  for (int k = 0; k < 5; ++k) b->lmax[k] = 0;
   for (int i = 0; i < nb; ++i) {
     const sdns_plan* p = b->box[i];
     const int M0 = p->M[0], M1 = p->M[1], n = p->n2h;
     const int l[5] = {M0, M1, n, M1, M0};
-    const int T1 = batch_T<Rt>(M0), T2 = batch_T<Rt>(M1), LPB = batch_LPB<Rt>(n);
     const int q[5] = {p->nky * p->nkzp, M0 * p->nkzp, M0 * M1, M0 * p->nkzop, p->N[1] * p->Nh};
-    const int per[5] = {T1, T2, LPB, T2, T1};
     for (int k = 0; k < 5; ++k) {
-      len[k].push_back(l[k]);
-      nt[k].push_back((q[k] + per[k] - 1) / per[k]);
-      wt[k].push_back(k == 2 ? batch_zE(l[k]) : batch_E(l[k]));
+      const int v = batch_variant(b, k, l[k]);
+      const int per = k == 2 ? batch_LPB<Rt>(l[k], v) : batch_T<Rt>(l[k], v);
+      len[k].push_back(l[k] + BATCH_VSTRIDE * v);
+      nt[k].push_back((q[k] + per - 1) / per);
+      wt[k].push_back(k == 2 ? batch_zE(l[k], v) : batch_E(l[k], v));
       b->lmax[k] = std::max(b->lmax[k], l[k]);
     }
   }
@@ -1433,6 +1443,21 @@ int sdns_batch_rk4_steps(sdns_batch* b, int nsteps, double* diag) {
     HIP_TRY(hipMemcpyAsync(diag, b->d_diag, 2 * b->nbox * sizeof(double), hipMemcpyDeviceToDevice, b->own));
   HIP_TRY(hipEventRecord(b->ev_out, b->own));
   HIP_TRY(hipStreamWaitEvent(b->user, b->ev_out, 0));
+  return 0;
+}
+
+int sdns_batch_set_variant(sdns_batch* b, int pass, int length, int variant) {
+  if (!b || pass < 0 || pass > 4 || length < 1 || variant < 0 || variant > 1)
+    return fail(SDNS_ERR_INVALID, "pass %d, length %d, variant %d", pass, length, variant);
+  b->sched[{pass, length}] = variant;
+  // new work lists: drain the batch stream, drop the captured step and lay out again
+  HIP_TRY(hipStreamSynchronize(b->own));
+  batch_drop_graph(b);
+  if (b->dmem) (void)hipFree(b->dmem);
+  b->dmem = nullptr;
+  const int rc = b->precision == SDNS_DOUBLE ? batch_layout<double>(b) : batch_layout<float>(b);
+  if (rc) return rc;
+  b->dirty = true;
   return 0;
 }
 

@@ -12,16 +12,19 @@
 
 namespace sdns {
 
-// Body of one case of the strided-pass length switch: line length m, tile T.
-#define SDNS_BATCH_CASE(m, ...)                         \
-  case m:                                               \
-    if constexpr (m <= LMAX) {                          \
-      constexpr int E = batch_E(m), T = batch_T<Rt>(m); \
-      C* tw = smem + m * T;                             \
-      table_to_lds(tw, (const C*)a.tw, m);              \
-      __VA_ARGS__;                                      \
-    }                                                   \
+// Cases of the strided-pass length switch: line length m with schedule
+// variant 0 (case m) and, where it differs, variant 1 (case m + BATCH_VSTRIDE),
+// tile T.
+#define SDNS_BATCH_CASE_V(m, V, ...)                          \
+  case m + V * BATCH_VSTRIDE:                                 \
+    if constexpr (m <= LMAX && (V == 0 || batch_E(m, 1) != batch_E(m, 0))) { \
+      constexpr int E = batch_E(m, V), T = batch_T<Rt>(m, V); \
+      C* tw = smem + m * T;                                   \
+      table_to_lds(tw, (const C*)a.tw, m);                    \
+      __VA_ARGS__;                                            \
+    }                                                         \
     break;
+#define SDNS_BATCH_CASE(m, ...) SDNS_BATCH_CASE_V(m, 0, __VA_ARGS__) SDNS_BATCH_CASE_V(m, 1, __VA_ARGS__)
 
 template <typename Rt, int LMAX>
 __global__ void __launch_bounds__(batch_threads<Rt>())
@@ -100,19 +103,21 @@ __global__ void __launch_bounds__(batch_zthreads<Rt>())
   if (w.box < 0) return;
   const P3Args<Rt> a = args[w.box];
   switch (len[w.box]) {
-#define SDNS_X(n)                                              \
-  case n:                                                      \
-    if constexpr (n <= LMAX) {                                 \
-      constexpr int E = batch_zE(n), LPB = batch_LPB<Rt>(n);   \
-      C* tw = smem + 3 * LPB * z3_slot(n, n / E, (int)sizeof(C)); \
-      C* rtw = tw + n;                                         \
-      table_to_lds(tw, (const C*)a.tw, n);                     \
-      table_to_lds(rtw, (const C*)a.rtw, n);                   \
-      p3b_body<Rt, n, E, LPB>(a, w.tile * LPB, smem, tw, rtw); \
-    }                                                          \
+#define SDNS_ZV(n, V)                                                    \
+  case n + V * BATCH_VSTRIDE:                                            \
+    if constexpr (n <= LMAX && (V == 0 || batch_zE(n, 1) != batch_zE(n, 0))) { \
+      constexpr int E = batch_zE(n, V), LPB = batch_LPB<Rt>(n, V);       \
+      C* tw = smem + 3 * LPB * z3_slot(n, n / E, (int)sizeof(C));        \
+      C* rtw = tw + n;                                                   \
+      table_to_lds(tw, (const C*)a.tw, n);                               \
+      table_to_lds(rtw, (const C*)a.rtw, n);                             \
+      p3b_body<Rt, n, E, LPB>(a, w.tile * LPB, smem, tw, rtw);           \
+    }                                                                    \
     break;
+#define SDNS_X(n) SDNS_ZV(n, 0) SDNS_ZV(n, 1)
     SDNS_BATCH_HALF(SDNS_X)
 #undef SDNS_X
+#undef SDNS_ZV
     default: break;
   }
 }

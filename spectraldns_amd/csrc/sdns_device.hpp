@@ -21,6 +21,8 @@
 #include <type_traits>
 #include <utility>
 
+#include "sdns_lineplan.hpp"
+
 namespace sdns {
 
 template <typename R> struct cx_of;
@@ -205,31 +207,6 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dft(C (&v)
   }
 }
 
-// ---------------------------------------------------------------- line plans
-// Radix of pass `pass` for a length-M line with E points per thread: the
-// largest divisor of E that divides what is left of M.
-constexpr int radix_at(int M, int E, int pass) {
-  int rem = M, r = 1;
-  for (int p = 0; p <= pass; ++p) {
-    if (rem == 1) return 1;
-    r = 1;
-    for (int d = E; d >= 2; --d)
-      if (E % d == 0 && rem % d == 0) { r = d; break; }
-    if (r == 1) return 0;  // not factorable with divisors of E
-    rem /= r;
-  }
-  return r;
-}
-constexpr int num_passes(int M, int E) {
-  int n = 0, rem = M;
-  while (rem > 1) {
-    int r = radix_at(M, E, n);
-    if (r <= 1) return -1;
-    rem /= r;
-    ++n;
-  }
-  return n;
-}
 
 // Points per thread.  Variant 0 is the default schedule; variant 1 is the
 // alternative the autotuner may pick (same as variant 0 where no other

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "sdns_lineplan.hpp"
 #include "sdns_types.hpp"
 
 // Line lengths with compiled kernels: strided (x, y) complex lines and the
@@ -92,34 +93,42 @@ template <typename Rt> int launch_errnorm(const void* err, const void* u0, const
 #define SDNS_BATCH_STRIDED(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128)
 #define SDNS_BATCH_HALF(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
 
-constexpr int batch_E(int M) {
+// Schedule variant v of a length: v = 0 is the default; v = 1 halves the
+// points per thread (twice the threads per line, half the columns or lines
+// per block, so no more LDS) where that line plan exists, and is the default
+// elsewhere.  A box's length reaches the kernels as m + BATCH_VSTRIDE * v.
+constexpr int BATCH_VSTRIDE = 8192;
+constexpr int batch_E0(int M) {
   switch (M) {
     case 16: return 4;  case 24: return 6;  case 32: return 8;  case 48: return 12;
     case 64: return 8;  case 96: return 12; case 128: return 16;
     default: return 0;
   }
 }
-constexpr int batch_zE(int n) {
+constexpr int batch_zE0(int n) {
   switch (n) {
     case 8: return 8; case 12: return 6; case 16: return 8; case 24: return 6;
     case 32: return 8; case 48: return 6; case 64: return 8;
     default: return 0;
   }
 }
+constexpr int batch_halved(int M, int e0) { return e0 >= 4 && num_passes(M, e0 / 2) > 0 ? e0 / 2 : e0; }
+constexpr int batch_E(int M, int v = 0) { return v ? batch_halved(M, batch_E0(M)) : batch_E0(M); }
+constexpr int batch_zE(int n, int v = 0) { return v ? batch_halved(n, batch_zE0(n)) : batch_zE0(n); }
 template <typename Rt> constexpr int batch_threads() { return sizeof(Rt) == 4 ? 256 : 128; }
 template <typename Rt> constexpr int batch_zthreads() { return sizeof(Rt) == 4 ? 128 : 64; }
-template <typename Rt> constexpr int batch_T(int M) {
-  return batch_E(M) ? batch_threads<Rt>() * batch_E(M) / M : 0;
+template <typename Rt> constexpr int batch_T(int M, int v = 0) {
+  return batch_E(M, v) ? batch_threads<Rt>() * batch_E(M, v) / M : 0;
 }
-template <typename Rt> constexpr int batch_LPB(int n) {
-  return batch_zE(n) ? batch_zthreads<Rt>() * batch_zE(n) / n : 0;
+template <typename Rt> constexpr int batch_LPB(int n, int v = 0) {
+  return batch_zE(n, v) ? batch_zthreads<Rt>() * batch_zE(n, v) / n : 0;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // LDS (complex elements) of a strided batch kernel compiled for lengths up
 // to lmax: tile + twiddle table
 template <typename Rt> constexpr int batch_strided_lds(int lmax) {
   int m = 0;
-#define SDNS_X(M) if (M <= lmax) m = cmax(m, M * batch_T<Rt>(M) + M);
+#define SDNS_X(M) if (M <= lmax) m = cmax(m, M * cmax(batch_T<Rt>(M, 0), batch_T<Rt>(M, 1)) + M);
   SDNS_BATCH_STRIDED(SDNS_X)
 #undef SDNS_X
   return m;
@@ -127,7 +136,10 @@ template <typename Rt> constexpr int batch_strided_lds(int lmax) {
 // LDS of the z batch kernel: three line slots (stride zslot) per line + two tables
 template <typename Rt> constexpr int batch_z_lds(int lmax) {
   int m = 0;
-#define SDNS_X(n) if (n <= lmax) m = cmax(m, 3 * batch_LPB<Rt>(n) * z3_slot(n, n / batch_zE(n), (int)sizeof(Rt) * 2) + 2 * n);
+#define SDNS_X(n)                                                                                         \
+  if (n <= lmax)                                                                                          \
+    for (int v = 0; v < 2; ++v)                                                                           \
+      m = cmax(m, 3 * batch_LPB<Rt>(n, v) * z3_slot(n, n / batch_zE(n, v), (int)sizeof(Rt) * 2) + 2 * n);
   SDNS_BATCH_HALF(SDNS_X)
 #undef SDNS_X
   return m;

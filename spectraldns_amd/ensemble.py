@@ -42,11 +42,12 @@ class Ensemble:
 
     init: "mixed" (even boxes Taylor-Green, odd boxes random phase), "tg" or
     "random".  Box i's random field uses seed + i; viscosities come from a
-    generator seeded with ``seed``.
+    generator seeded with ``seed``.  tuned: apply the ensemble autotuner's
+    stored schedules for these sizes.
     """
 
     def __init__(self, sizes=None, precision="single", dealias="2/3-rule", dealias_mode=0, seed=2024,
-                 dt=0.005, nu_range=(1e-3, 1e-2), init="mixed", k0=4.0, energy=0.5):
+                 dt=0.005, nu_range=(1e-3, 1e-2), init="mixed", k0=4.0, energy=0.5, tuned=True):
         import torch
         from ._abi import Batch
         self.sizes = list(default_sizes() if sizes is None else sizes)
@@ -70,6 +71,10 @@ class Ensemble:
                            dealias_mode)
         self.batch.set_state(self.u_hat)
         self.batch.set_params(self.nu, self.dt)
+        # the batched passes' schedules stored by the ensemble autotuner, if any
+        from .autotune import apply_batch_schedule
+        self.schedule = apply_batch_schedule(self.batch, self.sizes, precision, dealias, dealias_mode) \
+            if tuned else None
 
     def reset(self):
         """Copy the initial data back into the state tensors."""

@@ -166,3 +166,96 @@ def test_default_ensemble_96_boxes():
         assert rel(ens.u_hat[i].cpu().numpy(), ref) < 1e-5, i
         assert d[2 * i] < e0
         assert abs(d[2 * i] - g.kinetic_energy(ref)) <= 1e-5 * e0
+
+
+@pytest.mark.parametrize("rule,boxes", [("2/3-rule", BOXES_23), ("3/2-rule", BOXES_32)])
+def test_batch_schedule_variants_fp64(rule, boxes):
+    """Every (pass, length) schedule set to variant 1 (half the points per
+    thread, see sdns_batch_set_variant): FP64 boxes within 1e-12 of the oracle
+    after three steps and within 1e-13 of the default schedules, graph replay
+    after the new layout included; an invalid variant is rejected."""
+    from spectraldns_amd._abi import Batch, SdnsError
+    nsteps = 3
+    nus = [0.01 + 0.002 * i for i in range(len(boxes))]
+    dts = [0.004] * len(boxes)
+    u0, ref_gpu, _ = run_batch(boxes, "double", rule, nus, dts, nsteps)
+    print("\n[case] batch variants %s" % rule, flush=True)
+    u = [torch.from_numpy(a).to(torch.complex128).cuda().contiguous() for a in u0]
+    b = Batch(boxes, "double", rule, True, 0)
+    b.set_state(u)
+    b.set_params(nus, dts)
+    b.rk4_steps(1)                          # a captured step of the default layout first
+    for t, a in zip(u, u0):
+        t.copy_(torch.from_numpy(a))
+    for k, lengths in b.line_lengths().items():
+        for length in lengths:
+            b.set_variant(k, length, 1)
+    with pytest.raises(SdnsError):
+        b.set_variant(5, 32, 1)
+    b.rk4_steps(nsteps)
+    torch.cuda.synchronize()
+    for i, (N, L) in enumerate(boxes):
+        g = Grid(N, L, "double", rule)
+        ref = u0[i]
+        for _ in range(nsteps):
+            ref = g.rk4_step(ref, dts[i], nus[i])
+        got = u[i].cpu().numpy()
+        assert rel(got, ref) < 1e-12, (i, rel(got, ref))
+        assert rel(got, ref_gpu[i]) < 1e-13, (i, rel(got, ref_gpu[i]))
+
+
+@pytest.mark.timeout(300)
+def test_default_ensemble_500_steps():
+    """The ensemble benchmark configuration in full: 96 boxes (32 x 32^3,
+    32 x 48^3, 32 x 64^3), RK4, 2/3 rule, 500 steps (T = 2.5), the stored
+    schedules and graph replay of the bench.  Reference: the FP64 single-box
+    GPU path over the same 500 steps (held to 1e-12 of the oracle in the other
+    tests) and, for one 32^3 box, the oracle itself.
+      * FP64 batch: every box within 1e-11 relative L2 (the batched passes are
+        the single-box arithmetic, so only rounding differs);
+      * FP32 batch: the Taylor-Green boxes within 3e-6 relative L2, every box
+        within 5e-5, median below 5e-6, FP64 energy and dissipation within
+        1e-5.  The random-field boxes (k0 = 4, nu down to 1e-3) amplify FP32
+        rounding over the 2.5 time units (measured: Taylor-Green boxes
+        <= 1.6e-6, random boxes up to 3.7e-5 at 64^3, median 2.9e-6), so the
+        1e-5 bar of a single 256^3 run after a few steps is 5e-5 here after
+        500; the FP64 batch shows the batched arithmetic itself is exact."""
+    from spectraldns_amd._abi import Plan
+    from spectraldns_amd.ensemble import Ensemble
+    nsteps = 500
+    refs, plans = {}, {}
+    errs = {}
+    for precision in ("double", "single"):
+        print("\n[case] default ensemble, %s, 500 steps" % precision, flush=True)
+        ens = Ensemble(precision=precision)
+        diag = torch.zeros(2 * ens.n, dtype=torch.float64, device="cuda")
+        ens.steps(nsteps, diag)
+        torch.cuda.synchronize()
+        d = diag.cpu().numpy()
+        errs[precision] = []
+        for i, N in enumerate(ens.sizes):
+            if i not in refs:
+                p = plans.setdefault(N, Plan((N, N, N), (TWO_PI,) * 3, "double", "2/3-rule"))
+                ref = torch.from_numpy(ens.u0[i]).to(torch.complex128).cuda()
+                dd = torch.zeros(2 * nsteps, dtype=torch.float64, device="cuda")
+                p.rk4_steps(ref, [ens.dt[i]] * nsteps, ens.nu[i], diag=dd)
+                refs[i] = (ref.cpu().numpy(), dd[-2:].cpu().numpy())
+            ref, (e, eps) = refs[i]
+            errs[precision].append(rel(ens.u_hat[i].cpu().numpy().astype(np.complex128), ref))
+            tol = 1e-12 if precision == "double" else 1e-5
+            assert abs(d[2 * i] - e) <= tol * e and abs(d[2 * i + 1] - eps) <= tol * eps, (precision, i)
+        if precision == "double":
+            last = ens.u_hat[1].cpu().numpy()
+    e64, e32 = np.array(errs["double"]), np.array(errs["single"])
+    print("FP64 batch worst %.3e; FP32 batch worst %.3e (box %d), median %.3e" %
+          (e64.max(), e32.max(), int(e32.argmax()), float(np.median(e32))), flush=True)
+    assert e64.max() < 1e-11
+    tg = np.array([k == "tg" for k in ens.kind])
+    assert e32[tg].max() < 3e-6
+    assert e32.max() < 5e-5 and np.median(e32) < 5e-6
+    g = Grid((32, 32, 32), (TWO_PI,) * 3, "double", "2/3-rule")
+    ref = ens.u0[1]
+    for _ in range(nsteps):
+        ref = g.rk4_step(ref, ens.dt[1], ens.nu[1])
+    assert rel(last, ref) < 1e-11
+    assert rel(ens.u_hat[1].cpu().numpy().astype(np.complex128), ref) < 5e-5

@@ -298,3 +298,34 @@ def test_compensated_fp32_accumulation():
     print("after %d steps: plain fp32 %.3e, compensated %.3e" % (nsteps, e_plain, e_comp))
     assert e_comp < 0.5 * e_plain and e_comp < 2e-6
     assert abs(diag[-2].item() - g.kinetic_energy(ref)) < 1e-6 * g.kinetic_energy(ref)
+
+
+@pytest.mark.parametrize("N,rule", [((256, 16, 32), "2/3-rule"), ((256, 16, 16), "3/2-rule")])
+def test_x_forward_rk_variants(N, rule):
+    """Both schedules of the x-forward + RK pass on 256- and 384-point x lines
+    (the headline boxes' line lengths): two RK4 steps within 1e-12 of the
+    oracle in FP64 with the in-step energies, and the two FP32 schedules,
+    plain and compensated, within 1e-5 of the FP64 oracle."""
+    L = (TWO_PI, 3.0, 5.0)
+    nu, dt = 0.01, 0.002
+    g = Grid(N, L, "double", rule)
+    u = field(N, L, seed=31)
+    ref = g.rk4_step(g.rk4_step(u, dt, nu), dt, nu)
+    for var in range(2):
+        p = plan_of(N, L, "double", rule)
+        p.set_variant(4, var)
+        ud = to_gpu(u, torch.complex128)
+        diag = torch.zeros(4, dtype=torch.float64, device="cuda")
+        p.rk4_steps(ud, [dt, dt], nu, diag=diag)
+        assert rel(ud.cpu().numpy(), ref) < 1e-12, var
+        assert abs(diag[2].item() - g.kinetic_energy(ref)) < 1e-13 * g.kinetic_energy(ref), var
+        for comp in (False, True):
+            p = plan_of(N, L, "single", rule)
+            p.set_variant(4, var)
+            us = to_gpu(u, torch.complex64)
+            lo = torch.zeros_like(us) if comp else None
+            p.rk4_steps(us, [dt, dt], nu, u_lo=lo)
+            got = us.cpu().numpy().astype(np.complex128)
+            if comp:
+                got = got + lo.cpu().numpy().astype(np.complex128)
+            assert rel(got, ref) < 1e-5, (var, comp)

@@ -500,15 +500,17 @@ template <typename Rt> int convection(sdns_plan* p, const void* u, void* rhs, co
 template <typename Rt> EwArgs<Rt> ew_args(sdns_plan* p);
 
 // Network forward on the closure's packed input x0 -> y (FP32, 6 of 8 channels).
-int closure_net_forward(sdns_closure* c, hipStream_t st) {
+// keep_pre: store the pre-activations z1, z2 (the backward pass reads them;
+// inference does not).
+int closure_net_forward(sdns_closure* c, hipStream_t st, bool keep_pre) {
   ConvArgs a{};
   a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];
-  a.x = c->x0; a.w = c->w.w1f; a.bias = c->w.b1; a.z = c->z1; a.a = c->a1;
+  a.x = c->x0; a.w = c->w.w1f; a.bias = c->w.b1; a.z = keep_pre ? c->z1 : nullptr; a.a = c->a1;
   if (launch_conv(8, 32, CONV_EPI_SILU, a, st)) return fail(SDNS_ERR_UNSUPPORTED, "closure grid");
-  a.x = c->a1; a.w = c->w.w2f; a.bias = c->w.b2; a.z = c->z2; a.a = c->a2;
-  launch_conv(32, 32, CONV_EPI_SILU, a, st);
+  a.x = c->a1; a.w = c->w.w2f; a.bias = c->w.b2; a.z = keep_pre ? c->z2 : nullptr; a.a = c->a2;
+  if (launch_conv(32, 32, CONV_EPI_SILU, a, st)) return fail(SDNS_ERR_HIP, "closure layer 2");
   a.x = c->a2; a.w = c->w.w3f; a.bias = c->w.b3; a.z = nullptr; a.a = nullptr; a.y = c->y; a.ystride = 8; a.ncout = 6;
-  launch_conv(32, 16, CONV_EPI_LINEAR, a, st);
+  if (launch_conv(32, 16, CONV_EPI_LINEAR, a, st)) return fail(SDNS_ERR_HIP, "closure layer 3");
   return 0;
 }
 
@@ -518,7 +520,7 @@ int closure_source(sdns_closure* c, sdns_plan* p, const void* u, float in_scale,
   int rc;
   if ((rc = plain_backward<float>(p, u, c->phys, 3))) return rc;
   launch_pack_input(c->phys, in_scale, c->x0, c->P, p->stream);
-  if ((rc = closure_net_forward(c, p->stream))) return rc;
+  if ((rc = closure_net_forward(c, p->stream, false))) return rc;
   launch_unpack_output(c->y, out_scale, c->tauphys, c->P, p->stream);
   const size_t fbytes = (size_t)p->N[0] * p->N[1] * p->Nh * p->esz;
   if ((rc = plain_forward<float>(p, c->tauphys, c->tauhat, 3))) return rc;
@@ -1536,7 +1538,7 @@ int sdns_closure_forward(sdns_closure* c, const float* u_phys, float in_scale, f
   if (!c || !u_phys || !tau_phys) return fail(SDNS_ERR_INVALID, "bad argument");
   if (!c->have_params) return fail(SDNS_ERR_INVALID, "closure parameters not set");
   launch_pack_input(u_phys, in_scale, c->x0, c->P, c->stream);
-  int rc = closure_net_forward(c, c->stream);
+  int rc = closure_net_forward(c, c->stream, false);
   if (rc) return rc;
   launch_unpack_output(c->y, out_scale, tau_phys, c->P, c->stream);
   return check_launch();
@@ -1550,7 +1552,7 @@ int sdns_closure_train_step(sdns_closure* c, const float* u_phys, const float* t
   const long P = c->P;
   const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
   launch_pack_input(u_phys, in_scale, c->x0, P, st);
-  int rc = closure_net_forward(c, st);
+  int rc = closure_net_forward(c, st, true);
   if (rc) return rc;
   launch_pack_target(tau_phys, 1.0f / out_scale, c->t, P, st);
   launch_loss(c->y, c->t, c->gy, c->gyb, P, c->lpart, c->loss, st);
@@ -1562,13 +1564,13 @@ int sdns_closure_train_step(sdns_closure* c, const float* u_phys, const float* t
   ConvArgs a{};
   a.n0 = n0; a.n1 = n1; a.n2 = n2;
   a.x = c->gyb; a.w = c->w.w3b; a.zprev = c->z2; a.a = c->gz2;
-  launch_conv(8, 32, CONV_EPI_DSILU, a, st);
+  if (launch_conv(8, 32, CONV_EPI_DSILU, a, st)) return fail(SDNS_ERR_HIP, "closure layer 3 input gradient");
   // layer 2
   nch = launch_wgrad(32, 32, c->gz2, c->a1, c->wpart, n0, n1, n2, st);
   launch_wgrad_reduce(c->wpart, nch, 32, 32, 32, 32, grad + CL_OW2, st);
   launch_bgrad_bf16(c->gz2, 32, 32, P, c->bpart, grad + CL_OB2, st);
   a.x = c->gz2; a.w = c->w.w2b; a.zprev = c->z1; a.a = c->gz1;
-  launch_conv(32, 32, CONV_EPI_DSILU, a, st);
+  if (launch_conv(32, 32, CONV_EPI_DSILU, a, st)) return fail(SDNS_ERR_HIP, "closure layer 2 input gradient");
   // layer 1
   nch = launch_wgrad(32, 8, c->gz1, c->x0, c->wpart, n0, n1, n2, st);
   launch_wgrad_reduce(c->wpart, nch, 32, 8, 32, 3, grad + CL_OW1, st);

@@ -10,6 +10,8 @@
 // every A and B fragment one 16-byte load per lane.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sdns_closure.hpp"
 
 namespace sdns {
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
       const float v = acc[b][i] + bias;
       if constexpr (EPI == CONV_EPI_SILU) {
         const __bf16 zb = (__bf16)v;
-        a.z[pt * COUT + co] = zb;
+        if (a.z) a.z[pt * COUT + co] = zb;
         a.a[pt * COUT + co] = (__bf16)silu_f((float)zb);
       } else if constexpr (EPI == CONV_EPI_LINEAR) {
         if (co < a.ncout) a.y[pt * a.ystride + co] = v;
@@ -101,12 +103,182 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   }
 }
 
+// LDS-tiled convolution.  A block owns 4 consecutive y-rows x tzn = min(N2, 64)
+// consecutive z points over CONV_XP consecutive x-planes, and marches along
+// x: the layer's weights (all 27 taps) stay in LDS, and the periodic input
+// halo is a rolling window of x-planes (6 rows x (tzn + 2) points each) in 4
+// LDS slots, the plane after next loaded into registers while the current
+// output plane is computed and stored into its free slot afterwards.  Wave w
+// computes row w of the current plane: tzn / 16 point tiles x COUT / 16
+// channel tiles, each weight fragment read once per K-step and used for every
+// point tile.  The MFMA sequence per output is the one of k_conv (taps in the
+// same order, same K split), so the results are the same bit for bit.
+// Needs N0 % 4 == 0, N1 % 4 == 0, N2 % 16 == 0 and N2 <= 64 or N2 % 64 == 0.
+constexpr int CONV_ROWS = 4, CONV_XP = 4, CONV_SLOTS = 4;
+inline int conv_tzn(int n2) { return n2 <= 64 ? n2 : 64; }
+template <int CIN, int COUT> constexpr int conv_w_bytes() { return 27 * COUT * CIN * 2; }
+template <int CIN> constexpr int conv_plane_chunks(int tzn) { return (CONV_ROWS + 2) * (tzn + 2) * CIN / 8; }
+
+// A fragment (16 points x 8 channels per lane) of point tile t and the B
+// fragments of K-step ks for output row w; pl[tz] is the LDS slot of input
+// plane i0 - 1 + tz.  CIN = 32: one tap per K-step; CIN = 8: taps 4 ks ..
+// 4 ks + 3, one per lane group g (lanes past the last tap multiply zeros).
+template <int CIN, int COUT, int MT>
+__device__ __forceinline__ void conv_frags(int ks, const bf16x8* wl, const bf16x8* const (&pl)[3], int w, int m,
+                                           int g, bf16x8 (&av)[MT], bf16x8 (&bv)[COUT / 16]) {
+  constexpr int NB = COUT / 16, npt = 16 * MT + 2;
+  if constexpr (CIN == 32) {
+    const int tz = ks / 9, ty = (ks / 3) % 3, tx = ks % 3;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) bv[b] = wl[(ks * COUT + b * 16 + m) * 4 + g];
+    const bf16x8* h = pl[tz] + ((w + ty) * npt + m + tx) * 4 + g;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) av[t] = h[64 * t];
+  } else {
+    const bool ok = 4 * ks + g < 27;
+    const int tap = ok ? 4 * ks + g : 26;
+    const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) bv[b] = ok ? wl[tap * COUT + b * 16 + m] : bf16x8{};
+    const bf16x8* h = pl[tz] + (w + ty) * npt + m + tx;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) av[t] = ok ? h[16 * t] : bf16x8{};
+  }
+}
+
+template <int CIN, int COUT, int EPI, int MT>
+__global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
+  constexpr int NB = COUT / 16, CG = CIN / 8, KS = CIN == 32 ? 27 : 7;
+  constexpr int tzn = 16 * MT, npt = tzn + 2, PC = conv_plane_chunks<CIN>(tzn);
+  constexpr int PR = (PC + 255) / 256;  // prefetch registers (16-byte chunks) per thread
+  extern __shared__ __align__(16) unsigned char conv_smem[];
+  bf16x8* wl = (bf16x8*)conv_smem;
+  bf16x8* hl = (bf16x8*)(conv_smem + conv_w_bytes<CIN, COUT>());
+  const int n0 = a.n0, n1 = a.n1, n2 = a.n2;
+  const int nzb = n2 / tzn, nrb = n1 / CONV_ROWS;
+  int blk = blockIdx.x;
+  const int zb = blk % nzb;
+  blk /= nzb;
+  const int rb = blk % nrb, xc = blk / nrb;
+  const int i1b = rb * CONV_ROWS, z0 = zb * tzn, x0 = xc * CONV_XP;
+  const bf16x8* __restrict__ X = (const bf16x8*)a.x;
+  const bf16x8* __restrict__ W = (const bf16x8*)a.w;
+  // chunk c of an input plane: (row, point, channel group), row-major
+  auto load_plane = [&](int j, bf16x8 (&r)[PR]) {
+    const long pbase = (long)wrap(x0 + j, n0) * n1;
+#pragma unroll
+    for (int e = 0; e < PR; ++e) {
+      const int c = threadIdx.x + 256 * e;
+      if (c < PC) {
+        const int cg = c % CG, q = c / CG, pt = q % npt, row = q / npt;
+        r[e] = X[((pbase + wrap(i1b + row - 1, n1)) * n2 + wrap(z0 + pt - 1, n2)) * CG + cg];
+      }
+    }
+  };
+  auto store_plane = [&](int j, const bf16x8 (&r)[PR]) {
+    bf16x8* dst = hl + ((j + 1) % CONV_SLOTS) * PC;
+#pragma unroll
+    for (int e = 0; e < PR; ++e) {
+      const int c = threadIdx.x + 256 * e;
+      if (c < PC) dst[c] = r[e];
+    }
+  };
+  for (int c = threadIdx.x; c < 27 * COUT * CG; c += 256) wl[c] = W[c];
+  bf16x8 pre[PR];
+  for (int j = -1; j <= 1; ++j) {  // planes x0 - 1, x0, x0 + 1
+    load_plane(j, pre);
+    store_plane(j, pre);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  for (int k = 0; k < CONV_XP; ++k) {
+    if (k + 2 <= CONV_XP) load_plane(k + 2, pre);  // in flight during this plane's MFMAs
+    const bf16x8* const pl[3] = {hl + (k % CONV_SLOTS) * PC, hl + ((k + 1) % CONV_SLOTS) * PC,
+                                 hl + ((k + 2) % CONV_SLOTS) * PC};
+    f32x4 acc[MT][NB];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[t][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 av[MT], bv[NB], an[MT], bn[NB];
+    conv_frags<CIN, COUT, MT>(0, wl, pl, w, m, g, av, bv);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      if (ks + 1 < KS) conv_frags<CIN, COUT, MT>(ks + 1, wl, pl, w, m, g, an, bn);
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv[b], acc[t][b], 0, 0, 0);
+      if (ks + 1 < KS) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t) av[t] = an[t];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) bv[b] = bn[b];
+      }
+    }
+    // epilogue: D[4 g + i][m] of tile t -> point (x0 + k, i1b + w, z0 + 16 t + 4 g + i), channel 16 b + m
+    const long prow = ((long)(x0 + k) * n1 + i1b + w) * n2 + z0;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int co = b * 16 + m;
+        const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const long pt = prow + 16 * t + 4 * g + i;
+          const float v = acc[t][b][i] + bias;
+          if constexpr (EPI == CONV_EPI_SILU) {
+            const __bf16 zb = (__bf16)v;
+            if (a.z) a.z[pt * COUT + co] = zb;
+            a.a[pt * COUT + co] = (__bf16)silu_f((float)zb);
+          } else if constexpr (EPI == CONV_EPI_LINEAR) {
+            if (co < a.ncout) a.y[pt * a.ystride + co] = v;
+          } else {
+            a.a[pt * COUT + co] = (__bf16)(v * dsilu_f((float)a.zprev[pt * COUT + co]));
+          }
+        }
+      }
+    }
+    // the slot of plane k + 2 held plane k - 2, which no wave reads any more
+    if (k + 2 <= CONV_XP) store_plane(k + 2, pre);
+    __syncthreads();
+  }
+}
+
+template <int CIN, int COUT, int EPI, int MT> int conv_lds_go(const ConvArgs& a, hipStream_t s) {
+  const size_t bytes = (size_t)conv_w_bytes<CIN, COUT>() + (size_t)CONV_SLOTS * conv_plane_chunks<CIN>(16 * MT) * 16;
+  // dynamic LDS beyond 64 KB must be allowed for the kernel (on the current device)
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv_lds<CIN, COUT, EPI, MT>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return -1;
+  const unsigned nblk = (unsigned)((long)(a.n0 / CONV_XP) * (a.n1 / CONV_ROWS) * (a.n2 / (16 * MT)));
+  hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI, MT>), dim3(nblk), dim3(256), bytes, s, a);
+  return 0;
+}
+template <int CIN, int COUT, int EPI> int conv_lds_go(const ConvArgs& a, hipStream_t s) {
+  switch (conv_tzn(a.n2) / 16) {
+    case 1: return conv_lds_go<CIN, COUT, EPI, 1>(a, s);
+    case 2: return conv_lds_go<CIN, COUT, EPI, 2>(a, s);
+    case 3: return conv_lds_go<CIN, COUT, EPI, 3>(a, s);
+    default: return conv_lds_go<CIN, COUT, EPI, 4>(a, s);
+  }
+}
+
 int launch_conv(int cin, int cout, int epi, const ConvArgs& a, hipStream_t s) {
   const long P = (long)a.n0 * a.n1 * a.n2;
   if (a.n2 % 16 || P % 64) return -1;
+  // 32-channel inputs: the LDS-tiled kernel; 8-channel inputs (a quarter of the
+  // MFMAs of a 32-channel layer) run faster on k_conv's many small blocks
+  const bool tiled = cin == 32 && a.n0 % CONV_XP == 0 && a.n1 % CONV_ROWS == 0 && (a.n2 <= 64 || a.n2 % 64 == 0);
   const dim3 grid((unsigned)(P / 64)), blk(256);
-#define SDNS_CONV(CI, CO, EP) \
-  if (cin == CI && cout == CO && epi == EP) { hipLaunchKernelGGL((k_conv<CI, CO, EP>), grid, blk, 0, s, a); return 0; }
+#define SDNS_CONV(CI, CO, EP)                                                                       \
+  if (cin == CI && cout == CO && epi == EP) {                                                       \
+    if (tiled) return conv_lds_go<CI, CO, EP>(a, s);                                                \
+    hipLaunchKernelGGL((k_conv<CI, CO, EP>), grid, blk, 0, s, a);                                   \
+    return 0;                                                                                       \
+  }
   SDNS_CONV(8, 32, CONV_EPI_SILU)      // layer 1 forward
   SDNS_CONV(32, 32, CONV_EPI_SILU)     // layer 2 forward
   SDNS_CONV(32, 16, CONV_EPI_LINEAR)   // layer 3 forward

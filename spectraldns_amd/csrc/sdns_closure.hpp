@@ -41,7 +41,7 @@ struct ConvArgs {
   const __bf16* x;      // input [P][cin]
   const __bf16* w;      // [27][cout][cin]
   const float* bias;    // [cout] or null
-  __bf16* z;            // SILU: pre-activation out [P][cout]
+  __bf16* z;            // SILU: pre-activation out [P][cout] (null: not stored)
   __bf16* a;            // SILU: activation out; DSILU: gradient out [P][cout]
   float* y;             // LINEAR: FP32 out [P][ystride], channels < ncout written
   const __bf16* zprev;  // DSILU: stored pre-activation [P][cout] whose SiLU' multiplies the result
@@ -50,6 +50,10 @@ struct ConvArgs {
 };
 
 // cin in {8, 32}, cout in {16, 32}; N2 % 16 == 0 and N0*N1*N2 % 64 == 0.
+// 32-channel inputs are LDS-tiled (weights and a rolling window of input
+// planes staged per block) when N0 % 4 == 0, N1 % 4 == 0 and N2 <= 64 or
+// N2 % 64 == 0.  Returns nonzero if the shape or the launch set-up
+// is not supported.
 int launch_conv(int cin, int cout, int epi, const ConvArgs& a, hipStream_t s);
 // Weight-gradient partials: part[chunk][27][gc][xc] = sum over the chunk's
 // points p of g[p][co] x[p + d(tap)][ci]; gc, xc in {8, 32}; chunk = 4096

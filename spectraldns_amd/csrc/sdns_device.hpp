@@ -286,11 +286,21 @@ template <typename C> __device__ __forceinline__ void buf_store(C x, __amdgpu_bu
 // tw[m] = exp(-2 pi i m / M), m in [0, M): the forward twiddle table of the
 // line length; the inverse direction uses its conjugate.  WAVE: the P threads
 // of a line are in one wave (exchanges need no block barrier).
-template <typename Rt, int M, int E, int DIR, bool WAVE = false> struct LineFFT {
+//
+// First-exchange layout: with P = 16 threads per line and a first radix R0 = E
+// = 12 (the 192-point lines), the first pass writes element p = 12 t + k,
+// which puts the threads t, t + 4, t + 8, t + 12 of a line on the same LDS
+// banks.  SWZ stores element p of that exchange at p ^ ((p / 48) & 3): for the
+// writes (t = 4a + c) the XOR is a = t / 4 and the banks of the 16 threads
+// become distinct; for the reads p = t + 16 e it is the compile-time
+// (e / 3) & 3, so both sides keep immediate offsets from per-thread bases.
+template <typename Rt, int M, int E, int DIR, bool WAVE = false, bool SWZ = false> struct LineFFT {
   using C = cx<Rt>;
   static constexpr int P = M / E;
   static constexpr int NP = num_passes(M, E);
   static_assert(E > 0 && M % E == 0 && NP > 0, "unsupported line length");
+  static_assert(!SWZ || (P == 16 && radix_at(M, E, 0) == 12 && E == 12 && NP > 1),
+                "the first-exchange layout is derived for 16 threads and a first radix of 12");
 
   // lds: this line's exchange area, element p at lds[p * es].  Every thread
   // of the block (of the wave, if WAVE) must call run() (it contains barriers
@@ -321,7 +331,22 @@ template <typename Rt, int M, int E, int DIR, bool WAVE = false> struct LineFFT 
 #pragma unroll
       for (int k = 0; k < R; ++k) v[b + k * NB] = x[k];
     }
-    if constexpr (PASS + 1 < NP) {
+    if constexpr (SWZ && PASS == 0) {
+      // p = 12 t + k stored at 12 t + (k ^ a), a = (t / 4) & 3 (12 t is a
+      // multiple of 4, so the XOR stays in the low two bits of k); read back
+      // at 16 e + (t ^ ((e / 3) & 3))
+      const int a = (t >> 2) & 3;
+      C* wbase[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) wbase[c] = lds + (12 * t + (c ^ a)) * es;
+#pragma unroll
+      for (int k = 0; k < R; ++k) wbase[k & 3][4 * (k >> 2) * es] = v[k];
+      lds_barrier<WAVE>();
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[(16 * e + (t ^ ((e / 3) & 3))) * es];
+      lds_barrier<WAVE>();
+      pass<PASS + 1, NS * R>(v, t, lds, es, tw);
+    } else if constexpr (PASS + 1 < NP) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int j = t + P * b;

@@ -482,8 +482,10 @@ __device__ __forceinline__ void p3b_body(const P3Args<Rt>& a, int line0, cx<Rt>*
   constexpr int P = n / E, SL = z3_slot(n, P, (int)sizeof(cx<Rt>));
   constexpr int KS = stage_kept(n, P, NKI, LineStage<Rt, n, P>::K);
   constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
-  using Fi = LineFFT<Rt, n, E, +1, WL>;
-  using Ff = LineFFT<Rt, n, E, -1, WL>;
+  // the bank-conflict-free first exchange of the 192-point lines (12 points per thread)
+  constexpr bool SWZ = n == 192 && E == 12;
+  using Fi = LineFFT<Rt, n, E, +1, WL, SWZ>;
+  using Ff = LineFFT<Rt, n, E, -1, WL, SWZ>;
   __syncthreads();  // twiddle tables written by the caller
   const int t = threadIdx.x % P, l = threadIdx.x / P;
   C* s0 = lds + (3 * l) * SL;

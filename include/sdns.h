@@ -54,6 +54,10 @@ int sdns_plan_destroy(sdns_plan* plan);
  * runs).  The z pass (id 2) has variants 2 and 3 too (its LDS-assisted scheme
  * with schedule 0 / 1) and 4 and 5 (rows staged HBM -> LDS by direct loads,
  * schedule 0 / 1; where the rows do not fit in LDS these run variant 2 / 3).
+ * The x-forward + RK pass (id 4) has variants 2 and 3 too: schedules 0 / 1
+ * whose plain RK stages under the 3/2 rule use kernels with the kept x rows
+ * fixed at compile time (x lines of 96 to 768 points, arrays within 2 GiB;
+ * other cases run as variants 0 / 1).
  * Set by the autotuner (spectraldns_amd/autotune.py). */
 int sdns_plan_set_variant(sdns_plan* plan, int pass, int variant);
 /* Rebind the plan to another HIP stream (NULL = null stream). */

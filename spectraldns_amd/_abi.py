@@ -536,7 +536,8 @@ class Plan:
         return ub, tau
 
     def set_variant(self, pass_id, variant):
-        """Schedule variant (0 default, 1 alternative) of pass 0..4, 5 = plain transforms."""
+        """Schedule variant (0 default, 1 alternative) of pass 0..4, 5 = plain transforms;
+        the z pass (2) has variants 0..5 and the x-forward + RK pass (4) 0..3."""
         _check(self._lib.sdns_plan_set_variant(self._h, int(pass_id), int(variant)))
 
     # ---------------------------------------------------------------- timing

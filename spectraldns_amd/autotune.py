@@ -4,7 +4,9 @@ Every pass (0 x-inverse, 1 y-inverse + curl, 2 z + cross product, 3
 y-forward, 4 x-forward + RK stage) is compiled with up to two line-FFT
 schedules (points per thread and the radix sequence that follows from it,
 see sdns_device.hpp); the z pass also has three storage schemes (all in
-registers, LDS-assisted, LDS-DMA staging), six variants in all.  The tuner times each variant of each pass with HIP
+registers, LDS-assisted, LDS-DMA staging), six variants in all, and the
+x-forward + RK pass has its run-time row maps or its 3/2-rule stage kernels
+(kept rows fixed at compile time), four variants.  The tuner times each variant of each pass with HIP
 events inside real RK4 steps of the target box and keeps the fastest; the
 result is stored per box in ``schedules.json`` next to this file and applied
 by ``apply_schedule`` whenever a plan for that box is created.
@@ -62,7 +64,7 @@ def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, 
     res = {"variants": {}, "ms": {}}
     for pid, name in enumerate(PASSES):
         times = []
-        for var in ((0, 1, 2, 3, 4, 5) if pid == 2 else (0, 1)):
+        for var in ((0, 1, 2, 3, 4, 5) if pid == 2 else (0, 1, 2, 3) if pid == 4 else (0, 1)):
             plan.set_variant(pid, var)
             u = u0.clone()
             plan.rk4_steps(u, [1e-3], 1e-3)       # warm-up (first launch of this variant)

@@ -1049,7 +1049,7 @@ int sdns_plan_destroy(sdns_plan* p) {
 
 int sdns_plan_set_variant(sdns_plan* p, int pass, int variant) {
   if (!p) return fail(SDNS_ERR_INVALID, "null plan");
-  const int maxv = pass == 2 ? 5 : 1;
+  const int maxv = pass == 2 ? 5 : (pass == 4 ? 3 : 1);
   if (pass < 0 || pass > 5 || variant < 0 || variant > maxv)
     return fail(SDNS_ERR_INVALID, "pass %d variant %d", pass, variant);
   p->var[pass] = variant;

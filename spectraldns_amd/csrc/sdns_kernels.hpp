@@ -825,6 +825,26 @@ __device__ __forceinline__ void energy_terms(const double (&v)[6], Rt K0, Rt K1,
   osum += w * (ax * ax + ay * ay + bx * bx + by * by + cx * cx + cy * cy);
 }
 
+// Block sum of the per-thread FP64 (energy, enstrophy) sums of a P5 tile into
+// partials[2 tile], partials[2 tile + 1]; every thread of the block calls it.
+template <int TH> __device__ __forceinline__ void p5_partials(double esum, double osum, double* partials, int tile) {
+  __shared__ double red[2][TH / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    esum += __shfl_down(esum, off, 64);
+    osum += __shfl_down(osum, off, 64);
+  }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) { red[0][wave] = esum; red[1][wave] = osum; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0;
+    for (int w = 0; w < TH / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
+    partials[2 * tile] = s0;
+    partials[2 * tile + 1] = s1;
+  }
+}
+
 // COMP: compensated accumulation of the RK4 sum u2 = u1 + sum a dt rhs in a
 // (hi, lo) pair of single-precision fields; the state is u = hi + lo with hi
 // the caller's u_hat and lo the caller's u_lo.  Stage values stay plain.
@@ -1034,23 +1054,164 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
     }
   }
   if constexpr (MODE == P5_RK3) {
-    if (a.partials) {
-      __shared__ double red[2][TH / 64];
+    if (a.partials) p5_partials<TH>(esum, osum, a.partials, tile);
+  }
+}
+
+// P5 stages under the 3/2 rule (M = 3 N0 / 2 bins, kept output modes
+// -N0/2..N0/2-1).  The kept bins are p < M/3 (row p) and p >= 2M/3 (row
+// p - M/3); with p = t + P e that is e < E/3 and e >= 2E/3 on every lane, and
+// kept element j (e = j, or j + E/3 past the first third) lands in row
+// t + P j.  Known at compile time, these leave the update loop without
+// branches, and it is software-pipelined: the state rows of the next chunk of
+// G elements are loaded before the current chunk is computed and stored (an
+// element's rows are written only by that element, so in program order every
+// load precedes the stores it could meet).  Every access is a buffer access:
+// one descriptor per array, the lane's byte offset (the same for every
+// element) and a uniform offset of component and element in a scalar
+// register, so no access needs a 64-bit address of its own.  W4 entries of
+// columns that carry no kept mode get an offset past the descriptor and read
+// as zero.  Plain RK stages with no source and no pressure output; the
+// per-mode arithmetic is p5_body's.  The host checks that the three
+// components of a state array and of W4 span at most BUF_OOB bytes
+// (p5k_fits).
+template <typename Rt, int M, int E, int MODE, int T>
+__device__ __forceinline__ void p5k_body(const P5Args<Rt>& a, int tile, cx<Rt>* lds, const cx<Rt>* tw) {
+  using C = cx<Rt>;
+  static_assert(M % 3 == 0 && E % 3 == 0 && MODE != P5_RHS, "3/2-rule RK stages");
+  constexpr int P = M / E, TH = T * P, E3 = E / 3, NK = 2 * E3;
+  constexpr int CS = (int)sizeof(C);
+  // state arrays read per element: RK0 u1; RK1-2 u0, u1, u2; RK3 u0, u2
+  constexpr int NL = MODE == P5_RK12 ? 3 : (MODE == P5_RK3 ? 2 : 1);
+  constexpr int G = MODE == P5_RK12 ? 1 : 2;  // spill-free at E = 24 (256 VGPRs)
+  static_assert(NK % G == 0, "chunk must divide the kept elements");
+  using F = LineFFT<Rt, M, E, -1>;
+  const int c = threadIdx.x % T, t = threadIdx.x / T;
+  const int q = tile * T + c;
+  const bool valid = q < a.Q;
+  const int qq = valid ? q : 0;
+  const int iy = qq / a.Nh, kz = qq - iy * a.Nh;
+  const int ky = iy < a.N1 / 2 ? iy : iy - a.N1;
+  const int nkyo = a.y_neg + a.y_pos + 1;
+  const bool colconv = valid && ky >= -a.y_neg && ky <= a.y_pos && kz < a.nkzo;
+  const int kyc = ky >= 0 ? ky : nkyo + ky;
+  const int wrow = nkyo * a.nkzop, wfield = M * wrow;
+  const auto rw = buf_rsrc(a.w4, 3L * wfield * CS);
+  const int wo = colconv ? (t * wrow + kyc * a.nkzop + kz) * CS : BUF_OOB;
+  C X0[E], X1[E], X2[E];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        esum += __shfl_down(esum, off, 64);
-        osum += __shfl_down(osum, off, 64);
+  for (int e = 0; e < E; ++e) {
+    X0[e] = buf_load_so<C>(rw, wo, P * e * wrow * CS);
+    X1[e] = buf_load_so<C>(rw, wo, (wfield + P * e * wrow) * CS);
+    X2[e] = buf_load_so<C>(rw, wo, (2 * wfield + P * e * wrow) * CS);
+  }
+  __syncthreads();
+  C* line = lds + c;
+  F::run(X0, t, line, T, tw);
+  F::run(X1, t, line, T, tw);
+  F::run(X2, t, line, T, tw);
+
+  const int plane = a.N1 * a.Nh, fs = a.N0 * plane;
+  const Rt K1 = a.ky[iy], K2c = a.kz[kz];
+  const bool colok = colconv && !(a.mask_nyquist && (iy == a.N1 / 2 || kz == a.N2 / 2));
+  const int nyq_ix = a.mask_nyquist ? a.N0 / 2 : -1;
+  const double wz = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
+  double esum = 0.0, osum = 0.0;
+  const long span = 3L * fs * CS;
+  const auto r0p = buf_rsrc(MODE == P5_RK0 ? a.A : a.B, span);
+  const auto r1p = buf_rsrc(MODE == P5_RK12 ? a.A : a.Cacc, span);
+  const auto r2p = buf_rsrc(a.Cacc, span);
+  const auto rA = buf_rsrc(a.A, span);
+  const auto rB = buf_rsrc(a.B, span);
+  const auto rC = buf_rsrc(a.Cacc, span);
+  const Rt bdt = a.bdt, adt = a.adt;
+  const int so = (t * plane + q) * CS;  // row t of the column
+  if (valid) {
+    C cur[G][NL][3], nxt[G][NL][3];
+    Rt kcur[G], knxt[G];
+    auto load = [&](C(&d)[G][NL][3], Rt(&kd)[G], int j0) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int eo = P * (j0 + g) * plane;
+        kd[g] = a.kx[t + P * (j0 + g)];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          d[g][0][i] = buf_load_so<C>(r0p, so, (i * fs + eo) * CS);
+          if constexpr (NL > 1) d[g][1][i] = buf_load_so<C>(r1p, so, (i * fs + eo) * CS);
+          if constexpr (NL > 2) d[g][2][i] = buf_load_so<C>(r2p, so, (i * fs + eo) * CS);
+        }
       }
-      const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-      if (lane == 0) { red[0][wave] = esum; red[1][wave] = osum; }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double s0 = 0, s1 = 0;
-        for (int w = 0; w < TH / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
-        a.partials[2 * tile] = s0;
-        a.partials[2 * tile + 1] = s1;
+    };
+    load(cur, kcur, 0);
+#pragma unroll
+    for (int j0 = 0; j0 < NK; j0 += G) {
+      if (j0 + G < NK) load(nxt, knxt, j0 + G);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = j0 + g, e = j < E3 ? j : j + E3;
+        const int ix = t + P * j;
+        const int eo = P * j * plane;
+        const bool conv = colok & (ix != nyq_ix);
+        C r0 = conv ? cscale(X0[e], a.scale) : czero<C>();
+        C r1 = conv ? cscale(X1[e], a.scale) : czero<C>();
+        C r2 = conv ? cscale(X2[e], a.scale) : czero<C>();
+        const Rt K0 = kcur[g];
+        Rt k2 = K0 * K0;
+        k2 += K1 * K1;
+        k2 += K2c * K2c;
+        const Rt den = k2 == Rt(0) ? Rt(1) : k2;
+        const Rt o0 = K0 / den, o1 = K1 / den, o2 = K2c / den;
+        // P_hat = sum_i rhs_i K_i/K^2 ; rhs_i -= P_hat K_i ; rhs -= (nu K^2) u_hat  (NS.py:208-215)
+        C ph = mk<C>(r0.x * o0 + r1.x * o1, r0.y * o0 + r1.y * o1);
+        ph = mk<C>(ph.x + r2.x * o2, ph.y + r2.y * o2);
+        r0 = mk<C>(r0.x - ph.x * K0, r0.y - ph.y * K0);
+        r1 = mk<C>(r1.x - ph.x * K1, r1.y - ph.y * K1);
+        r2 = mk<C>(r2.x - ph.x * K2c, r2.y - ph.y * K2c);
+        const Rt z = a.nu * k2;
+        const C s0 = cur[g][0][0], s1 = cur[g][0][1], s2 = cur[g][0][2];
+        r0 = mk<C>(r0.x - z * s0.x, r0.y - z * s0.y);
+        r1 = mk<C>(r1.x - z * s1.x, r1.y - z * s1.y);
+        r2 = mk<C>(r2.x - z * s2.x, r2.y - z * s2.y);
+        auto upd = [](C base, C r, Rt s) { return mk<C>(base.x + s * r.x, base.y + s * r.y); };
+        const int o0s = eo * CS, o1s = (fs + eo) * CS, o2s = (2 * fs + eo) * CS;
+        if constexpr (MODE == P5_RK0) {
+          // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
+          buf_store_so<C>(upd(s0, r0, bdt), rB, so, o0s);
+          buf_store_so<C>(upd(s1, r1, bdt), rB, so, o1s);
+          buf_store_so<C>(upd(s2, r2, bdt), rB, so, o2s);
+          buf_store_so<C>(upd(s0, r0, adt), rC, so, o0s);
+          buf_store_so<C>(upd(s1, r1, adt), rC, so, o1s);
+          buf_store_so<C>(upd(s2, r2, adt), rC, so, o2s);
+        } else if constexpr (MODE == P5_RK12) {
+          buf_store_so<C>(upd(cur[g][1][0], r0, bdt), rB, so, o0s);
+          buf_store_so<C>(upd(cur[g][1][1], r1, bdt), rB, so, o1s);
+          buf_store_so<C>(upd(cur[g][1][2], r2, bdt), rB, so, o2s);
+          buf_store_so<C>(upd(cur[g][2][0], r0, adt), rC, so, o0s);
+          buf_store_so<C>(upd(cur[g][2][1], r1, adt), rC, so, o1s);
+          buf_store_so<C>(upd(cur[g][2][2], r2, adt), rC, so, o2s);
+        } else {
+          const C n0 = upd(cur[g][1][0], r0, adt), n1 = upd(cur[g][1][1], r1, adt), n2 = upd(cur[g][1][2], r2, adt);
+          buf_store_so<C>(n0, rA, so, o0s);
+          buf_store_so<C>(n1, rA, so, o1s);
+          buf_store_so<C>(n2, rA, so, o2s);
+          const double v[6] = {n0.x, n0.y, n1.x, n1.y, n2.x, n2.y};
+          energy_terms(v, K0, K1, K2c, wz, esum, osum);
+        }
+      }
+      if (j0 + G < NK) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          kcur[g] = knxt[g];
+#pragma unroll
+          for (int l = 0; l < NL; ++l)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) cur[g][l][i] = nxt[g][l][i];
+        }
       }
     }
+  }
+  if constexpr (MODE == P5_RK3) {
+    if (a.partials) p5_partials<TH>(esum, osum, a.partials, tile);
   }
 }
 
@@ -1079,6 +1240,18 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p5_occ<Rt, M, E>()
   if (tile < 0) return;
   table_to_lds(tw, (const C*)a.tw, M);
   p5_body<Rt, M, E, MODE, COMP, T>(a, tile, lds, tw);
+}
+// The 3/2-rule RK stages (p5k_body).
+template <typename Rt, int M, int E, int MODE>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p5_occ<Rt, M, E>())) k_p5k(P5Args<Rt> a) {
+  using C = cx<Rt>;
+  constexpr int T = SCfg<Rt, M, E>::T;
+  __shared__ C lds[M * T];
+  __shared__ C tw[M];
+  const int tile = xcd_tile((a.Q + T - 1) / T);
+  if (tile < 0) return;
+  table_to_lds(tw, (const C*)a.tw, M);
+  p5k_body<Rt, M, E, MODE, T>(a, tile, lds, tw);
 }
 
 }  // namespace sdns

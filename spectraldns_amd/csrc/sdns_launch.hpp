@@ -22,7 +22,8 @@ namespace sdns {
 template <typename Rt> int launch_p1(int M, int var, const P1Args<Rt>& a, hipStream_t s);
 template <typename Rt> int launch_p2(int M, int var, const P2Args<Rt>& a, hipStream_t s);
 template <typename Rt> int launch_p3(int n, int var, const P3Args<Rt>& a, hipStream_t s);
-// P5: returns the number of column tiles in *nblk (entries of the partial sums)
+// P5: returns the number of column tiles in *nblk (entries of the partial sums);
+// var 2 and 3 run schedules 0 and 1 with the 3/2-rule stage kernels where they apply
 template <typename Rt> int launch_p5(int M, int var, int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk);
 // dir = -1 forward, +1 inverse
 template <typename Rt> int launch_strided(int M, int var, int dir, int nfields, const StridedArgs<Rt>& a,

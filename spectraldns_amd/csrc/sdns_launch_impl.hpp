@@ -58,11 +58,34 @@ template <typename Rt, int n, int E> void p3c_go(const P3Args<Rt>& a, hipStream_
   if (p3c_try<Rt, n, E, 4>(a, s)) return;
   p3b_go<Rt, n, E>(a, s);  // rows too long for the LDS budget: LDS-assisted scheme
 }
-template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk) {
+// The 3/2-rule P5 stages (k_p5k) are compiled for the line lengths of the
+// padded grids from 96 to 768 points; schedule variants 2 and 3 run them when
+// the stage is a plain RK stage of the 3/2 rule whose arrays a buffer
+// descriptor reaches (otherwise, and for variants 0 and 1, p5_body runs).
+constexpr bool p5k_built(int M, int E) { return M % 3 == 0 && E % 3 == 0 && M >= 96 && M <= 768; }
+template <typename Rt> bool p5k_fits(int M, int mode, const P5Args<Rt>& a) {
+  if (mode == P5_RHS || a.Alo || a.source || a.phat || a.conv_only) return false;
+  if (3 * a.N0 != 2 * M || a.x.neg != a.N0 / 2 || a.x.pos != a.N0 / 2 - 1 || a.x.base != a.N0) return false;
+  const long esz = 2 * (long)sizeof(Rt);
+  const long state = 3L * a.N0 * a.N1 * a.Nh * esz;
+  const long w4 = 3L * M * (a.y_neg + a.y_pos + 1) * a.nkzop * esz;
+  return state <= BUF_OOB && w4 <= BUF_OOB;
+}
+template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, hipStream_t s, int* nblk, bool fixed_rows) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
   if (nblk) *nblk = nt;
   const dim3 g(xcd_grid(nt)), b(Cfg::THREADS);
+  if constexpr (p5k_built(M, E)) {
+    if (fixed_rows && p5k_fits(M, mode, a)) {
+      switch (mode) {
+        case P5_RK0: hipLaunchKernelGGL((k_p5k<Rt, M, E, P5_RK0>), g, b, 0, s, a); break;
+        case P5_RK12: hipLaunchKernelGGL((k_p5k<Rt, M, E, P5_RK12>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_p5k<Rt, M, E, P5_RK3>), g, b, 0, s, a); break;
+      }
+      return;
+    }
+  }
   if constexpr (sizeof(Rt) == 4) {
     if (a.Alo && mode != P5_RHS) {
       switch (mode) {
@@ -145,8 +168,8 @@ template <typename Rt> int launch_p5(int M, int var, int mode, const P5Args<Rt>&
   switch (M) {
 #define SDNS_X(m)                                                        \
   case m:                                                                \
-    if (var) p5_go<Rt, m, strided_E(m, 1)>(mode, a, s, nblk);            \
-    else p5_go<Rt, m, strided_E(m, 0)>(mode, a, s, nblk);                \
+    if (var & 1) p5_go<Rt, m, strided_E(m, 1)>(mode, a, s, nblk, var >= 2); \
+    else p5_go<Rt, m, strided_E(m, 0)>(mode, a, s, nblk, var >= 2);      \
     return 0;
     SDNS_STRIDED_SIZES(SDNS_X)
 #undef SDNS_X

@@ -118,6 +118,41 @@ def test_rk4_fp64_ten_steps(N, L, rule, mode):
     assert np.max(abs(dg - np.array(E)) / np.abs(np.array(E))) < 1e-12
 
 
+@pytest.mark.parametrize("mask", [True, False])
+@pytest.mark.parametrize("N,L", [((64, 48, 32), (TWO_PI, 3.0, 5.0)), ((96, 64, 32), (4 * np.pi, TWO_PI, 3.0))])
+def test_rk4_32rule_stages(N, L, mask):
+    """RK4 stages under the 3/2 rule on x lines of 96 and 144 points, with the
+    run-time row maps (x-forward variant 0) and with the stage kernels whose
+    kept x rows are fixed at compile time (variants 2 and 3): ten steps in
+    FP64 within 1e-12 of the oracle with the per-step energy and dissipation,
+    with and without the Nyquist mask, and FP32 within 1e-5 of the FP64
+    oracle."""
+    nu = 0.005
+    g = Grid(N, L, "double", "3/2-rule", mask)
+    u = field(N, L, seed=17)
+    dts = dt_schedule(0.1, 0.01)
+    got = {}
+    for var in (0, 2, 3):
+        p = plan_of(N, L, "double", "3/2-rule", mask=mask)
+        p.set_variant(4, var)
+        ud = to_gpu(u, torch.complex128)
+        diag = torch.zeros(2 * len(dts), dtype=torch.float64, device="cuda")
+        p.rk4_steps(ud, dts, nu, diag=diag)
+        ps = plan_of(N, L, "single", "3/2-rule", mask=mask)
+        ps.set_variant(4, var)
+        us = to_gpu(u, torch.complex64)
+        ps.rk4_steps(us, dts, np.float32(nu))
+        got[var] = (ud.cpu().numpy(), diag.cpu().numpy().reshape(-1, 2), us.cpu().numpy().astype(np.complex128))
+    E = []
+    for d in dts:
+        u = g.rk4_step(u, d, nu)
+        E.append((g.kinetic_energy(u), g.dissipation(u, nu)))
+    for var, (u64, dg, u32) in got.items():
+        assert rel(u64, u) < 1e-12, var
+        assert np.max(abs(dg - np.array(E)) / np.abs(np.array(E))) < 1e-12, var
+        assert rel(u32, u) < 1e-5, var
+
+
 @pytest.mark.parametrize("N,L,rule", [((32, 32, 32), (TWO_PI,) * 3, "2/3-rule"),
                                       ((32, 32, 32), (TWO_PI,) * 3, "3/2-rule"),
                                       ((64, 32, 16), (6 * np.pi, 4 * np.pi, 2 * np.pi), "2/3-rule")])
@@ -302,23 +337,30 @@ def test_compensated_fp32_accumulation():
 
 @pytest.mark.parametrize("N,rule", [((256, 16, 32), "2/3-rule"), ((256, 16, 16), "3/2-rule")])
 def test_x_forward_rk_variants(N, rule):
-    """Both schedules of the x-forward + RK pass on 256- and 384-point x lines
-    (the headline boxes' line lengths): two RK4 steps within 1e-12 of the
-    oracle in FP64 with the in-step energies, and the two FP32 schedules,
-    plain and compensated, within 1e-5 of the FP64 oracle."""
+    """Every variant of the x-forward + RK pass on 256- and 384-point x lines
+    (the headline boxes' line lengths): both schedules, and under the 3/2 rule
+    both the run-time row maps (variants 0, 1) and the stage kernels with the
+    kept rows fixed at compile time (2, 3).  Two RK4 steps within 1e-12 of
+    the oracle in FP64 with the in-step energies, the two row handlings of a
+    schedule within 1e-15 of each other, and FP32, plain and compensated,
+    within 1e-5 of the FP64 oracle."""
     L = (TWO_PI, 3.0, 5.0)
     nu, dt = 0.01, 0.002
     g = Grid(N, L, "double", rule)
     u = field(N, L, seed=31)
     ref = g.rk4_step(g.rk4_step(u, dt, nu), dt, nu)
-    for var in range(2):
+    got64 = []
+    for var in range(4):
         p = plan_of(N, L, "double", rule)
         p.set_variant(4, var)
         ud = to_gpu(u, torch.complex128)
         diag = torch.zeros(4, dtype=torch.float64, device="cuda")
         p.rk4_steps(ud, [dt, dt], nu, diag=diag)
-        assert rel(ud.cpu().numpy(), ref) < 1e-12, var
+        got64.append(ud.cpu().numpy())
+        assert rel(got64[-1], ref) < 1e-12, var
         assert abs(diag[2].item() - g.kinetic_energy(ref)) < 1e-13 * g.kinetic_energy(ref), var
+        if var >= 2:
+            assert rel(got64[var], got64[var - 2]) < 1e-15, var
         for comp in (False, True):
             p = plan_of(N, L, "single", rule)
             p.set_variant(4, var)

@@ -152,8 +152,17 @@ __device__ __forceinline__ void p1_body(const P1Args<Rt>& a, int tile, int yb, c
   store(comp, v);
 }
 
+// Waves per SIMD the x-inverse kernel is compiled for: the register estimate,
+// raised to 3 for FP32 when three blocks fit in a CU's LDS (the 24-point
+// schedule of the 384-point lines then fits 168 VGPRs with 16 bytes of spills
+// per lane; at 2 waves per SIMD it waits on its stores).
+template <typename Rt, int M, int E> constexpr int p1_occ() {
+  constexpr int o = occ_for(2 * E, esz_of<Rt>());
+  constexpr int lds = (M * SCfg<Rt, M, E>::T + M) * 2 * esz_of<Rt>();
+  return sizeof(Rt) == 4 && o < 3 && 163840 / lds >= 3 && 3 * SCfg<Rt, M, E>::THREADS <= 4 * 64 * 3 ? 3 : o;
+}
 template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(2 * E, esz_of<Rt>()))) k_p1(P1Args<Rt> a) {
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p1_occ<Rt, M, E>())) k_p1(P1Args<Rt> a) {
   using C = cx<Rt>;
   constexpr int T = SCfg<Rt, M, E>::T;
   __shared__ C lds[M * T];

@@ -31,6 +31,14 @@ template <> struct cx_of<double> { using type = double2; };
 template <typename R> using cx = typename cx_of<R>::type;
 
 template <typename C> __device__ __forceinline__ C mk(decltype(C::x) a, decltype(C::x) b) { C r; r.x = a; r.y = b; return r; }
+// Store of a pass output that the next pass reads once: non-temporal.
+template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
+  typedef decltype(C::x) V __attribute__((ext_vector_type(2)));
+  V w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
@@ -290,6 +298,15 @@ template <typename C> __device__ __forceinline__ void buf_store_so(C x, __amdgpu
   } else {
     static_assert(sizeof(C) == 16, "complex of float or double");
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, voff, soff, 0);
+  }
+}
+// Non-temporal variant (cache policy nt) for pass outputs read once by the next pass.
+template <typename C> __device__ __forceinline__ void buf_store_nt(C x, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(C) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u32x2, x), r, off, 0, 2);
+  } else {
+    static_assert(sizeof(C) == 16, "complex of float or double");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, off, 0, 2);
   }
 }
 template <typename C> __device__ __forceinline__ void buf_store(C x, __amdgpu_buffer_rsrc_t r, int off) {

@@ -209,7 +209,7 @@ __device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, c
   auto store = [&](int f, C(&v)[E]) {
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) out[f * ofield + (long)(t + P * e) * rs] = v[e];
+      for (int e = 0; e < E; ++e) st_stream(out + f * ofield + (long)(t + P * e) * rs, v[e]);
     }
   };
   if (yb == 0) {
@@ -375,8 +375,8 @@ __device__ __forceinline__ void r2c_post_buf(cx<Rt> (&z)[E], cx<Rt>* my, __amdgp
   for (int e = 0; e < EHI; ++e) {
     const int k = t + P * e;
     const C X = r2c_bin(z[e], my[(n - k) % n], rtw[k]);
-    buf_store<C>(X, rs, t < nk - P * e ? base + ES * k : BUF_OOB);
-    if (k == 0 && nk > n) buf_store<C>(mk<C>(z[e].x - z[e].y, Rt(0)), rs, base + ES * n);
+    buf_store_nt<C>(X, rs, t < nk - P * e ? base + ES * k : BUF_OOB);
+    if (k == 0 && nk > n) buf_store_nt<C>(mk<C>(z[e].x - z[e].y, Rt(0)), rs, base + ES * n);
   }
   lds_barrier<WAVE>();
 }
@@ -794,7 +794,7 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int r = bin_row(t + P * e, M, a.rout);
-      if (r >= 0) out[(long)r * a.out_row] = cscale(v[e], a.scale);
+      if (r >= 0) st_stream(out + (long)r * a.out_row, cscale(v[e], a.scale));
     }
   }
 }
@@ -993,15 +993,15 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
         if constexpr (!COMP) {
           if constexpr (MODE == P5_RK0) {
             // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
-            B[off] = upd(s0, r0, bdt); B[fs + off] = upd(s1, r1, bdt); B[2 * fs + off] = upd(s2, r2, bdt);
-            Cc[off] = upd(s0, r0, adt); Cc[fs + off] = upd(s1, r1, adt); Cc[2 * fs + off] = upd(s2, r2, adt);
+            st_stream(B + (off), upd(s0, r0, bdt)); st_stream(B + (fs + off), upd(s1, r1, bdt)); st_stream(B + (2 * fs + off), upd(s2, r2, bdt));
+            st_stream(Cc + (off), upd(s0, r0, adt)); st_stream(Cc + (fs + off), upd(s1, r1, adt)); st_stream(Cc + (2 * fs + off), upd(s2, r2, adt));
           } else if constexpr (MODE == P5_RK12) {
-            B[off] = upd(xa[g][0], r0, bdt); B[fs + off] = upd(xa[g][1], r1, bdt); B[2 * fs + off] = upd(xa[g][2], r2, bdt);
-            Cc[off] = upd(xc[g][0], r0, adt); Cc[fs + off] = upd(xc[g][1], r1, adt);
-            Cc[2 * fs + off] = upd(xc[g][2], r2, adt);
+            st_stream(B + (off), upd(xa[g][0], r0, bdt)); st_stream(B + (fs + off), upd(xa[g][1], r1, bdt)); st_stream(B + (2 * fs + off), upd(xa[g][2], r2, bdt));
+            st_stream(Cc + (off), upd(xc[g][0], r0, adt)); st_stream(Cc + (fs + off), upd(xc[g][1], r1, adt));
+            st_stream(Cc + (2 * fs + off), upd(xc[g][2], r2, adt));
           } else {
             const C n0 = upd(xc[g][0], r0, adt), n1 = upd(xc[g][1], r1, adt), n2 = upd(xc[g][2], r2, adt);
-            A[off] = n0; A[fs + off] = n1; A[2 * fs + off] = n2;
+            st_stream(A + (off), n0); st_stream(A + (fs + off), n1); st_stream(A + (2 * fs + off), n2);
             if (a.phat) ((C*)a.phat)[off] = ph;
             if (a.partials) {
               const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
@@ -1018,24 +1018,24 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               const long o = i * fs + off;
-              B[o] = upd(ss[i], rr[i], bdt);
+              st_stream(B + (o), upd(ss[i], rr[i], bdt));
               const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
               Rt ex, ey;
               const Rt hx = two_sum(ss[i].x, yx, ex), hy = two_sum(ss[i].y, yy, ey);
               const C l = xl[g][i];
-              Cc[o] = mk<C>(hx, hy);
+              st_stream(Cc + (o), mk<C>(hx, hy));
               Clo[o] = mk<C>(l.x + ex, l.y + ey);
             }
           } else if constexpr (MODE == P5_RK12) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               const long o = i * fs + off;
-              B[o] = upd(xa[g][i], rr[i], bdt);
+              st_stream(B + (o), upd(xa[g][i], rr[i], bdt));
               const C c = xc[g][i], l = xl[g][i];
               const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
               Rt ex, ey;
               const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
-              Cc[o] = mk<C>(hx, hy);
+              st_stream(Cc + (o), mk<C>(hx, hy));
               Clo[o] = mk<C>(l.x + ex, l.y + ey);
             }
           } else {
@@ -1050,7 +1050,7 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
               const Rt lx = l.x + ex, ly = l.y + ey;
               Rt rx, ry;  // renormalise: hi = fl(h + l), lo = exact remainder
               const Rt nx = two_sum(hx, lx, rx), ny = two_sum(hy, ly, ry);
-              A[o] = mk<C>(nx, ny);
+              st_stream(A + (o), mk<C>(nx, ny));
               Alo[o] = mk<C>(rx, ry);
               v[2 * i] = (double)nx + (double)rx;
               v[2 * i + 1] = (double)ny + (double)ry;

@@ -284,6 +284,8 @@ template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc
 }
 // With a uniform part of the offset in a scalar register (soff): the lane's
 // voff + soff addresses the element; the descriptor must cover voff + soff.
+// The store is non-temporal (the RK stage values are read once, by the next
+// stage).
 template <typename C> __device__ __forceinline__ C buf_load_so(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
   if constexpr (sizeof(C) == 8) {
     return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
@@ -292,12 +294,12 @@ template <typename C> __device__ __forceinline__ C buf_load_so(__amdgpu_buffer_r
     return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
   }
 }
-template <typename C> __device__ __forceinline__ void buf_store_so(C x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+template <typename C> __device__ __forceinline__ void buf_store_so_nt(C x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
   if constexpr (sizeof(C) == 8) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u32x2, x), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u32x2, x), r, voff, soff, 2);
   } else {
     static_assert(sizeof(C) == 16, "complex of float or double");
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, voff, soff, 2);
   }
 }
 // Non-temporal variant (cache policy nt) for pass outputs read once by the next pass.
@@ -307,14 +309,6 @@ template <typename C> __device__ __forceinline__ void buf_store_nt(C x, __amdgpu
   } else {
     static_assert(sizeof(C) == 16, "complex of float or double");
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, off, 0, 2);
-  }
-}
-template <typename C> __device__ __forceinline__ void buf_store(C x, __amdgpu_buffer_rsrc_t r, int off) {
-  if constexpr (sizeof(C) == 8) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u32x2, x), r, off, 0, 0);
-  } else {
-    static_assert(sizeof(C) == 16, "complex of float or double");
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(buf_u32x4, x), r, off, 0, 0);
   }
 }
 

@@ -348,8 +348,8 @@ __device__ __forceinline__ void r2c_post(cx<Rt> (&z)[E], cx<Rt>* my, cx<Rt>* __r
   for (int e = 0; e < E; ++e) {
     const int k = t + P * e;
     const C X = r2c_bin(z[e], my[(n - k) % n], rtw[k]);
-    if (valid && t < nk - P * e) out[k] = cscale(X, scale);
-    if (valid && k == 0 && nk > n) out[n] = mk<C>((z[e].x - z[e].y) * scale, Rt(0));
+    if (valid && t < nk - P * e) st_stream(out + k, cscale(X, scale));
+    if (valid && k == 0 && nk > n) st_stream(out + n, mk<C>((z[e].x - z[e].y) * scale, Rt(0)));
   }
   lds_barrier<WAVE>();
 }
@@ -738,7 +738,7 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   if (valid) {
     C* out = (C*)a.out + blockIdx.y * a.out_field + li * a.out_line;  // real line viewed as n complex pairs
 #pragma unroll
-    for (int e = 0; e < E; ++e) out[t + P * e] = cscale(z[e], a.scale);
+    for (int e = 0; e < E; ++e) st_stream(out + t + P * e, cscale(z[e], a.scale));
   }
 }
 
@@ -950,9 +950,9 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
         if (a.conv_only) {
           const bool cv = colconv && kept;
           C* rhs = (C*)a.rhs;
-          rhs[off] = cv ? cscale(X0[e], a.scale) : czero<C>();
-          rhs[fs + off] = cv ? cscale(X1[e], a.scale) : czero<C>();
-          rhs[2 * fs + off] = cv ? cscale(X2[e], a.scale) : czero<C>();
+          st_stream(rhs + off, cv ? cscale(X0[e], a.scale) : czero<C>());
+          st_stream(rhs + fs + off, cv ? cscale(X1[e], a.scale) : czero<C>());
+          st_stream(rhs + 2 * fs + off, cv ? cscale(X2[e], a.scale) : czero<C>());
           continue;
         }
       }
@@ -982,7 +982,7 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
       }
       if constexpr (MODE == P5_RHS) {
         C* rhs = (C*)a.rhs;
-        rhs[off] = r0; rhs[fs + off] = r1; rhs[2 * fs + off] = r2;
+        st_stream(rhs + off, r0); st_stream(rhs + fs + off, r1); st_stream(rhs + 2 * fs + off, r2);
         if (a.phat) ((C*)a.phat)[off] = ph;
       } else {
         C* A = (C*)a.A;
@@ -1185,24 +1185,24 @@ __device__ __forceinline__ void p5k_body(const P5Args<Rt>& a, int tile, cx<Rt>* 
         const int o0s = eo * CS, o1s = (fs + eo) * CS, o2s = (2 * fs + eo) * CS;
         if constexpr (MODE == P5_RK0) {
           // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
-          buf_store_so<C>(upd(s0, r0, bdt), rB, so, o0s);
-          buf_store_so<C>(upd(s1, r1, bdt), rB, so, o1s);
-          buf_store_so<C>(upd(s2, r2, bdt), rB, so, o2s);
-          buf_store_so<C>(upd(s0, r0, adt), rC, so, o0s);
-          buf_store_so<C>(upd(s1, r1, adt), rC, so, o1s);
-          buf_store_so<C>(upd(s2, r2, adt), rC, so, o2s);
+          buf_store_so_nt<C>(upd(s0, r0, bdt), rB, so, o0s);
+          buf_store_so_nt<C>(upd(s1, r1, bdt), rB, so, o1s);
+          buf_store_so_nt<C>(upd(s2, r2, bdt), rB, so, o2s);
+          buf_store_so_nt<C>(upd(s0, r0, adt), rC, so, o0s);
+          buf_store_so_nt<C>(upd(s1, r1, adt), rC, so, o1s);
+          buf_store_so_nt<C>(upd(s2, r2, adt), rC, so, o2s);
         } else if constexpr (MODE == P5_RK12) {
-          buf_store_so<C>(upd(cur[g][1][0], r0, bdt), rB, so, o0s);
-          buf_store_so<C>(upd(cur[g][1][1], r1, bdt), rB, so, o1s);
-          buf_store_so<C>(upd(cur[g][1][2], r2, bdt), rB, so, o2s);
-          buf_store_so<C>(upd(cur[g][2][0], r0, adt), rC, so, o0s);
-          buf_store_so<C>(upd(cur[g][2][1], r1, adt), rC, so, o1s);
-          buf_store_so<C>(upd(cur[g][2][2], r2, adt), rC, so, o2s);
+          buf_store_so_nt<C>(upd(cur[g][1][0], r0, bdt), rB, so, o0s);
+          buf_store_so_nt<C>(upd(cur[g][1][1], r1, bdt), rB, so, o1s);
+          buf_store_so_nt<C>(upd(cur[g][1][2], r2, bdt), rB, so, o2s);
+          buf_store_so_nt<C>(upd(cur[g][2][0], r0, adt), rC, so, o0s);
+          buf_store_so_nt<C>(upd(cur[g][2][1], r1, adt), rC, so, o1s);
+          buf_store_so_nt<C>(upd(cur[g][2][2], r2, adt), rC, so, o2s);
         } else {
           const C n0 = upd(cur[g][1][0], r0, adt), n1 = upd(cur[g][1][1], r1, adt), n2 = upd(cur[g][1][2], r2, adt);
-          buf_store_so<C>(n0, rA, so, o0s);
-          buf_store_so<C>(n1, rA, so, o1s);
-          buf_store_so<C>(n2, rA, so, o2s);
+          buf_store_so_nt<C>(n0, rA, so, o0s);
+          buf_store_so_nt<C>(n1, rA, so, o1s);
+          buf_store_so_nt<C>(n2, rA, so, o2s);
           const double v[6] = {n0.x, n0.y, n1.x, n1.y, n2.x, n2.y};
           energy_terms(v, K0, K1, K2c, wz, esum, osum);
         }

@@ -31,6 +31,16 @@ template <> struct cx_of<double> { using type = double2; };
 template <typename R> using cx = typename cx_of<R>::type;
 
 template <typename C> __device__ __forceinline__ C mk(decltype(C::x) a, decltype(C::x) b) { C r; r.x = a; r.y = b; return r; }
+// Load of a pass input that is read once, where neighbouring tiles do not
+// share its 128-byte lines: non-temporal.
+template <typename C> __device__ __forceinline__ C ld_stream(const C* p) {
+  typedef decltype(C::x) V __attribute__((ext_vector_type(2)));
+  const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+  C r;
+  r.x = w.x;
+  r.y = w.y;
+  return r;
+}
 // Store of a pass output that the next pass reads once: non-temporal.
 template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
   typedef decltype(C::x) V __attribute__((ext_vector_type(2)));

@@ -203,7 +203,7 @@ __device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, c
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int r = row(e);
-      v[e] = r >= 0 ? in[f * ifield + (long)r * rs] : czero<C>();
+      v[e] = r >= 0 ? ld_stream(in + f * ifield + (long)r * rs) : czero<C>();
     }
   };
   auto store = [&](int f, C(&v)[E]) {
@@ -786,7 +786,7 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int r = bin_row(t + P * e, M, a.rin);
-    v[e] = (valid && r >= 0) ? in[(long)r * a.in_row] : czero<C>();
+    v[e] = (valid && r >= 0) ? ld_stream(in + (long)r * a.in_row) : czero<C>();
   }
   __syncthreads();
   F::run(v, t, lds + c, T, tw);

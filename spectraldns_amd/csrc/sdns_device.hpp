@@ -49,6 +49,17 @@ template <typename C> __device__ __forceinline__ void st_stream(C* p, C v) {
   w.y = v.y;
   __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
 }
+// Pass inputs and outputs of column-tiled kernels: non-temporal when the
+// block's tile covers whole 128-byte lines (STREAM); narrower tiles share
+// lines with their neighbours through L2 and stay cached.
+template <bool STREAM, typename C> __device__ __forceinline__ C ld_pass(const C* p) {
+  if constexpr (STREAM) return ld_stream(p);
+  else return *p;
+}
+template <bool STREAM, typename C> __device__ __forceinline__ void st_pass(C* p, C v) {
+  if constexpr (STREAM) st_stream(p, v);
+  else *p = v;
+}
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {

@@ -180,6 +180,7 @@ __global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p1_occ<Rt, M, E>()
 template <typename Rt, int M, int E, int T>
 __device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, cx<Rt>* lds, const cx<Rt>* tw) {
   using C = cx<Rt>;
+  constexpr bool STREAM = T * (int)sizeof(C) >= 128;  // tiles of whole 128-byte lines
   constexpr int P = M / E;
   using F = LineFFT<Rt, M, E, +1>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
@@ -203,13 +204,13 @@ __device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, c
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int r = row(e);
-      v[e] = r >= 0 ? ld_stream(in + f * ifield + (long)r * rs) : czero<C>();
+      v[e] = r >= 0 ? ld_pass<STREAM>(in + f * ifield + (long)r * rs) : czero<C>();
     }
   };
   auto store = [&](int f, C(&v)[E]) {
     if (valid) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) st_stream(out + f * ofield + (long)(t + P * e) * rs, v[e]);
+      for (int e = 0; e < E; ++e) st_pass<STREAM>(out + f * ofield + (long)(t + P * e) * rs, v[e]);
     }
   };
   if (yb == 0) {
@@ -774,6 +775,7 @@ template <typename Rt, int M, int E, int DIR, int T>
 __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile, int yb, cx<Rt>* lds,
                                              const cx<Rt>* tw) {
   using C = cx<Rt>;
+  constexpr bool STREAM = T * (int)sizeof(C) >= 128;  // tiles of whole 128-byte lines
   constexpr int P = M / E;
   using F = LineFFT<Rt, M, E, DIR>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
@@ -786,7 +788,7 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int r = bin_row(t + P * e, M, a.rin);
-    v[e] = (valid && r >= 0) ? ld_stream(in + (long)r * a.in_row) : czero<C>();
+    v[e] = (valid && r >= 0) ? ld_pass<STREAM>(in + (long)r * a.in_row) : czero<C>();
   }
   __syncthreads();
   F::run(v, t, lds + c, T, tw);
@@ -794,7 +796,7 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int r = bin_row(t + P * e, M, a.rout);
-      if (r >= 0) st_stream(out + (long)r * a.out_row, cscale(v[e], a.scale));
+      if (r >= 0) st_pass<STREAM>(out + (long)r * a.out_row, cscale(v[e], a.scale));
     }
   }
 }
@@ -860,6 +862,7 @@ template <int TH> __device__ __forceinline__ void p5_partials(double esum, doubl
 template <typename Rt, int M, int E, int MODE, bool COMP, int T>
 __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* lds, const cx<Rt>* tw) {
   using C = cx<Rt>;
+  constexpr bool STREAM = T * (int)sizeof(C) >= 128;  // tiles of whole 128-byte lines
   constexpr int P = M / E, TH = T * P;
   using F = LineFFT<Rt, M, E, -1>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
@@ -950,9 +953,9 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
         if (a.conv_only) {
           const bool cv = colconv && kept;
           C* rhs = (C*)a.rhs;
-          st_stream(rhs + off, cv ? cscale(X0[e], a.scale) : czero<C>());
-          st_stream(rhs + fs + off, cv ? cscale(X1[e], a.scale) : czero<C>());
-          st_stream(rhs + 2 * fs + off, cv ? cscale(X2[e], a.scale) : czero<C>());
+          st_pass<STREAM>(rhs + off, cv ? cscale(X0[e], a.scale) : czero<C>());
+          st_pass<STREAM>(rhs + fs + off, cv ? cscale(X1[e], a.scale) : czero<C>());
+          st_pass<STREAM>(rhs + 2 * fs + off, cv ? cscale(X2[e], a.scale) : czero<C>());
           continue;
         }
       }
@@ -982,7 +985,7 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
       }
       if constexpr (MODE == P5_RHS) {
         C* rhs = (C*)a.rhs;
-        st_stream(rhs + off, r0); st_stream(rhs + fs + off, r1); st_stream(rhs + 2 * fs + off, r2);
+        st_pass<STREAM>(rhs + off, r0); st_pass<STREAM>(rhs + fs + off, r1); st_pass<STREAM>(rhs + 2 * fs + off, r2);
         if (a.phat) ((C*)a.phat)[off] = ph;
       } else {
         C* A = (C*)a.A;
@@ -993,15 +996,15 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
         if constexpr (!COMP) {
           if constexpr (MODE == P5_RK0) {
             // u0 = u1 + b dt rhs ; u2 = u1 + a dt rhs  (u1 = u2 = u0 at stage 0)
-            st_stream(B + (off), upd(s0, r0, bdt)); st_stream(B + (fs + off), upd(s1, r1, bdt)); st_stream(B + (2 * fs + off), upd(s2, r2, bdt));
-            st_stream(Cc + (off), upd(s0, r0, adt)); st_stream(Cc + (fs + off), upd(s1, r1, adt)); st_stream(Cc + (2 * fs + off), upd(s2, r2, adt));
+            st_pass<STREAM>(B + (off), upd(s0, r0, bdt)); st_pass<STREAM>(B + (fs + off), upd(s1, r1, bdt)); st_pass<STREAM>(B + (2 * fs + off), upd(s2, r2, bdt));
+            st_pass<STREAM>(Cc + (off), upd(s0, r0, adt)); st_pass<STREAM>(Cc + (fs + off), upd(s1, r1, adt)); st_pass<STREAM>(Cc + (2 * fs + off), upd(s2, r2, adt));
           } else if constexpr (MODE == P5_RK12) {
-            st_stream(B + (off), upd(xa[g][0], r0, bdt)); st_stream(B + (fs + off), upd(xa[g][1], r1, bdt)); st_stream(B + (2 * fs + off), upd(xa[g][2], r2, bdt));
-            st_stream(Cc + (off), upd(xc[g][0], r0, adt)); st_stream(Cc + (fs + off), upd(xc[g][1], r1, adt));
-            st_stream(Cc + (2 * fs + off), upd(xc[g][2], r2, adt));
+            st_pass<STREAM>(B + (off), upd(xa[g][0], r0, bdt)); st_pass<STREAM>(B + (fs + off), upd(xa[g][1], r1, bdt)); st_pass<STREAM>(B + (2 * fs + off), upd(xa[g][2], r2, bdt));
+            st_pass<STREAM>(Cc + (off), upd(xc[g][0], r0, adt)); st_pass<STREAM>(Cc + (fs + off), upd(xc[g][1], r1, adt));
+            st_pass<STREAM>(Cc + (2 * fs + off), upd(xc[g][2], r2, adt));
           } else {
             const C n0 = upd(xc[g][0], r0, adt), n1 = upd(xc[g][1], r1, adt), n2 = upd(xc[g][2], r2, adt);
-            st_stream(A + (off), n0); st_stream(A + (fs + off), n1); st_stream(A + (2 * fs + off), n2);
+            st_pass<STREAM>(A + (off), n0); st_pass<STREAM>(A + (fs + off), n1); st_pass<STREAM>(A + (2 * fs + off), n2);
             if (a.phat) ((C*)a.phat)[off] = ph;
             if (a.partials) {
               const double w = (kz == 0 || kz == a.N2 / 2) ? 1.0 : 2.0;
@@ -1018,24 +1021,24 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               const long o = i * fs + off;
-              st_stream(B + (o), upd(ss[i], rr[i], bdt));
+              st_pass<STREAM>(B + (o), upd(ss[i], rr[i], bdt));
               const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
               Rt ex, ey;
               const Rt hx = two_sum(ss[i].x, yx, ex), hy = two_sum(ss[i].y, yy, ey);
               const C l = xl[g][i];
-              st_stream(Cc + (o), mk<C>(hx, hy));
+              st_pass<STREAM>(Cc + (o), mk<C>(hx, hy));
               Clo[o] = mk<C>(l.x + ex, l.y + ey);
             }
           } else if constexpr (MODE == P5_RK12) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               const long o = i * fs + off;
-              st_stream(B + (o), upd(xa[g][i], rr[i], bdt));
+              st_pass<STREAM>(B + (o), upd(xa[g][i], rr[i], bdt));
               const C c = xc[g][i], l = xl[g][i];
               const Rt yx = adt * rr[i].x, yy = adt * rr[i].y;
               Rt ex, ey;
               const Rt hx = two_sum(c.x, yx, ex), hy = two_sum(c.y, yy, ey);
-              st_stream(Cc + (o), mk<C>(hx, hy));
+              st_pass<STREAM>(Cc + (o), mk<C>(hx, hy));
               Clo[o] = mk<C>(l.x + ex, l.y + ey);
             }
           } else {
@@ -1050,7 +1053,7 @@ __device__ __forceinline__ void p5_body(const P5Args<Rt>& a, int tile, cx<Rt>* l
               const Rt lx = l.x + ex, ly = l.y + ey;
               Rt rx, ry;  // renormalise: hi = fl(h + l), lo = exact remainder
               const Rt nx = two_sum(hx, lx, rx), ny = two_sum(hy, ly, ry);
-              st_stream(A + (o), mk<C>(nx, ny));
+              st_pass<STREAM>(A + (o), mk<C>(nx, ny));
               Alo[o] = mk<C>(rx, ry);
               v[2 * i] = (double)nx + (double)rx;
               v[2 * i + 1] = (double)ny + (double)ry;

@@ -15,6 +15,9 @@
 // Every kernel takes E (points per thread) as a template parameter so the
 // launcher can pick one of the compiled schedules per line length.
 #pragma once
+#include <cstdint>
+#include <type_traits>
+
 #include "sdns_device.hpp"
 #include "sdns_types.hpp"
 
@@ -775,7 +778,6 @@ template <typename Rt, int M, int E, int DIR, int T>
 __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile, int yb, cx<Rt>* lds,
                                              const cx<Rt>* tw) {
   using C = cx<Rt>;
-  constexpr bool STREAM = T * (int)sizeof(C) >= 128;  // tiles of whole 128-byte lines
   constexpr int P = M / E;
   using F = LineFFT<Rt, M, E, DIR>;
   const int c = threadIdx.x % T, t = threadIdx.x / T;
@@ -785,19 +787,27 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
   const C* __restrict__ in = (const C*)a.in + yb * a.in_field + (valid ? hi * a.in_hi + lo : 0);
   C* __restrict__ out = (C*)a.out + yb * a.out_field + hi * a.out_hi + lo;
   C v[E];
+  auto load = [&](auto stream) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int r = bin_row(t + P * e, M, a.rin);
-    v[e] = (valid && r >= 0) ? ld_pass<STREAM>(in + (long)r * a.in_row) : czero<C>();
-  }
+    for (int e = 0; e < E; ++e) {
+      const int r = bin_row(t + P * e, M, a.rin);
+      v[e] = (valid && r >= 0) ? ld_pass<decltype(stream)::value>(in + (long)r * a.in_row) : czero<C>();
+    }
+  };
+  if (a.stream_in) load(std::true_type{});
+  else load(std::false_type{});
   __syncthreads();
   F::run(v, t, lds + c, T, tw);
-  if (valid) {
+  auto store = [&](auto stream) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int r = bin_row(t + P * e, M, a.rout);
-      if (r >= 0) st_pass<STREAM>(out + (long)r * a.out_row, cscale(v[e], a.scale));
+      if (r >= 0) st_pass<decltype(stream)::value>(out + (long)r * a.out_row, cscale(v[e], a.scale));
     }
+  };
+  if (valid) {
+    if (a.stream_out) store(std::true_type{});
+    else store(std::false_type{});
   }
 }
 

@@ -103,10 +103,20 @@ template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, h
     default: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK3>), g, b, 0, s, a); break;
   }
 }
-template <typename Rt, int M, int E> void strided_go(int dir, int nfields, const StridedArgs<Rt>& a, hipStream_t s) {
+// Every column tile of T columns at element offsets hi * hs + lo (lo < cdiv),
+// rows rs and fields fs apart, covers whole 128-byte lines of `p`.
+template <typename Rt> bool tiles_whole_lines(const void* p, int T, long cdiv, long hs, long rs, long fs) {
+  const long esz = 2 * (long)sizeof(Rt);
+  return T * esz % 128 == 0 && cdiv % T == 0 && hs * esz % 128 == 0 && rs * esz % 128 == 0 &&
+         fs * esz % 128 == 0 && reinterpret_cast<uintptr_t>(p) % 128 == 0;
+}
+template <typename Rt, int M, int E> void strided_go(int dir, int nfields, const StridedArgs<Rt>& a0, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
-  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
+  const int nt = (a0.Q + Cfg::T - 1) / Cfg::T;
   const dim3 g(xcd_grid(nt), nfields), b(Cfg::THREADS);
+  StridedArgs<Rt> a = a0;
+  a.stream_in = tiles_whole_lines<Rt>(a.in, Cfg::T, a.cdiv, a.in_hi, a.in_row, a.in_field);
+  a.stream_out = tiles_whole_lines<Rt>(a.out, Cfg::T, a.cdiv, a.out_hi, a.out_row, a.out_field);
   if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, M, E, -1>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_strided<Rt, M, E, 1>), g, b, 0, s, a);
 }

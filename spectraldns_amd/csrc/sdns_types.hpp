@@ -74,6 +74,8 @@ template <typename Rt> struct StridedArgs {
   long out_hi, out_row, out_field;
   RowMap rin, rout;
   Rt scale;           // applied to every output (1 = none)
+  int stream_in, stream_out;  // non-temporal loads / stores (set by the launcher
+                              // when every column tile covers whole 128-byte lines)
 };
 
 // z-axis real transforms on contiguous lines: C2R (spectral lines of nk_in

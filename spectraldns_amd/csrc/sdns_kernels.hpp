@@ -572,9 +572,13 @@ __global__ void __launch_bounds__((ZBCfg<Rt, n, E>::THREADS), (occ_for(4 * E + s
   using C = cx<Rt>;
   constexpr int LPB = ZBCfg<Rt, n, E>::LPB;
   __shared__ C lds[3 * LPB * z3_slot(n, n / E, (int)sizeof(cx<Rt>))];
-  __shared__ C tw[n], rtw[n];
+  // FP64: the real-transform twiddles are read from global memory (a 3 KB
+  // table that stays in L1) so that one more block fits in a CU's LDS
+  constexpr bool RTW_LDS = z3_rtw_in_lds((int)sizeof(cx<Rt>));
+  __shared__ C tw[n], rtw_s[RTW_LDS ? n : 1];
   table_to_lds(tw, (const C*)a.tw, n);
-  table_to_lds(rtw, (const C*)a.rtw, n);
+  if constexpr (RTW_LDS) table_to_lds(rtw_s, (const C*)a.rtw, n);
+  const C* rtw = RTW_LDS ? rtw_s : (const C*)a.rtw;
   p3b_body<Rt, n, E, LPB, NKI, NKO>(a, blockIdx.x * LPB, lds, tw, rtw);
 }
 

@@ -140,8 +140,12 @@ constexpr int z3_lines(int n, int P, int esz, int sl) {
   while (L > 1 && 3 * L * sl * esz > 40960) L /= 2;
   return L;
 }
+// The single-box kernel keeps the real-transform twiddles in LDS for FP32
+// and reads them from global memory for FP64 (the batched kernel keeps both
+// tables in LDS).
+constexpr bool z3_rtw_in_lds(int esz) { return esz == 8; }
 constexpr int z3_blocks_per_cu(int n, int P, int esz, int sl) {
-  return 163840 / ((3 * z3_lines(n, P, esz, sl) * sl + 2 * n) * esz);
+  return 163840 / ((3 * z3_lines(n, P, esz, sl) * sl + (z3_rtw_in_lds(esz) ? 2 * n : n)) * esz);
 }
 constexpr int z3_slot(int n, int P, int esz) {
   return z3_lines(n, P, esz, zslot(n, P)) == z3_lines(n, P, esz, n + 1) &&

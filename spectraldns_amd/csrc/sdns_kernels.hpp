@@ -164,10 +164,11 @@ template <typename Rt, int M, int E> constexpr int p1_occ() {
   constexpr int lds = (M * SCfg<Rt, M, E>::T + M) * 2 * esz_of<Rt>();
   return sizeof(Rt) == 4 && o < 3 && 163840 / lds >= 3 && 3 * SCfg<Rt, M, E>::THREADS <= 4 * 64 * 3 ? 3 : o;
 }
-template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p1_occ<Rt, M, E>())) k_p1(P1Args<Rt> a) {
+// D: the launcher's tile divisor for small grids (T / D columns per block).
+template <typename Rt, int M, int E, int D = 1>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS / D), (p1_occ<Rt, M, E>())) k_p1(P1Args<Rt> a) {
   using C = cx<Rt>;
-  constexpr int T = SCfg<Rt, M, E>::T;
+  constexpr int T = SCfg<Rt, M, E>::T / D;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
@@ -255,10 +256,10 @@ __device__ __forceinline__ void p2_body(const P2Args<Rt>& a, int tile, int yb, c
   }
 }
 
-template <typename Rt, int M, int E>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(3 * E, esz_of<Rt>()))) k_p2(P2Args<Rt> a) {
+template <typename Rt, int M, int E, int D = 1>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS / D), (occ_for(3 * E, esz_of<Rt>()))) k_p2(P2Args<Rt> a) {
   using C = cx<Rt>;
-  constexpr int T = SCfg<Rt, M, E>::T;
+  constexpr int T = SCfg<Rt, M, E>::T / D;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
@@ -419,11 +420,11 @@ struct LineStage {
 };
 
 // ------------------------------------------------------------------ P3
-template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(6 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3(P3Args<Rt> a) {
+template <typename Rt, int n, int E, int D = 1>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS / D), (occ_for(6 * E + stage_k(n, n / E), esz_of<Rt>()))) k_p3(P3Args<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
-  constexpr int P = Cfg::P, LPB = Cfg::LPB;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB / D;
   constexpr bool WL = 64 % P == 0;  // a line's threads share one wave
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
@@ -717,11 +718,11 @@ __global__ void __launch_bounds__(LPB * (n / E), 2) k_p3c(P3Args<Rt> a) {
 }
 
 // ------------------------------------------------------------------ plain z transforms
-template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_c2r(ZArgs<Rt> a) {
+template <typename Rt, int n, int E, int D = 1>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS / D), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_c2r(ZArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
-  constexpr int P = Cfg::P, LPB = Cfg::LPB;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB / D;
   constexpr bool WL = 64 % P == 0;
   using Fi = LineFFT<Rt, n, E, +1, WL>;
   __shared__ C lds[LPB * zslot(n, P)];
@@ -750,11 +751,11 @@ __global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_
   }
 }
 
-template <typename Rt, int n, int E>
-__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_r2c(ZArgs<Rt> a) {
+template <typename Rt, int n, int E, int D = 1>
+__global__ void __launch_bounds__((ZCfg<Rt, n, E>::THREADS / D), (occ_for(E + stage_k(n, n / E), esz_of<Rt>()))) k_z_r2c(ZArgs<Rt> a) {
   using C = cx<Rt>;
   using Cfg = ZCfg<Rt, n, E>;
-  constexpr int P = Cfg::P, LPB = Cfg::LPB;
+  constexpr int P = Cfg::P, LPB = Cfg::LPB / D;
   constexpr bool WL = 64 % P == 0;
   using Ff = LineFFT<Rt, n, E, -1, WL>;
   __shared__ C lds[LPB * zslot(n, P)];
@@ -815,10 +816,10 @@ __device__ __forceinline__ void strided_body(const StridedArgs<Rt>& a, int tile,
   }
 }
 
-template <typename Rt, int M, int E, int DIR>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (occ_for(E, esz_of<Rt>()))) k_strided(StridedArgs<Rt> a) {
+template <typename Rt, int M, int E, int DIR, int D = 1>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS / D), (occ_for(E, esz_of<Rt>()))) k_strided(StridedArgs<Rt> a) {
   using C = cx<Rt>;
-  constexpr int T = SCfg<Rt, M, E>::T;
+  constexpr int T = SCfg<Rt, M, E>::T / D;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);
@@ -1256,10 +1257,10 @@ template <typename Rt, int M, int E> constexpr int p5_occ() {
   const int o = occ_blocks(occ_for(3 * E, esz_of<Rt>()), SCfg<Rt, M, E>::THREADS);
   return sizeof(Rt) == 4 && o < 2 ? 2 : o;
 }
-template <typename Rt, int M, int E, int MODE, bool COMP = false>
-__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS), (p5_occ<Rt, M, E>())) k_p5(P5Args<Rt> a) {
+template <typename Rt, int M, int E, int MODE, bool COMP = false, int D = 1>
+__global__ void __launch_bounds__((SCfg<Rt, M, E>::THREADS / D), (p5_occ<Rt, M, E>())) k_p5(P5Args<Rt> a) {
   using C = cx<Rt>;
-  constexpr int T = SCfg<Rt, M, E>::T;
+  constexpr int T = SCfg<Rt, M, E>::T / D;
   __shared__ C lds[M * T];
   __shared__ C tw[M];
   const int tile = xcd_tile((a.Q + T - 1) / T);

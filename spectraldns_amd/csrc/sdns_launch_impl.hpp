@@ -1,25 +1,64 @@
 // Launcher bodies (size dispatch) and elementwise kernels.  Included once per
 // working precision by sdns_f32.hip / sdns_f64.hip.
 #pragma once
+#include <type_traits>
+
 #include "sdns_kernels.hpp"
 #include "sdns_launch.hpp"
 
 namespace sdns {
 
 // ---------------------------------------------------------------- launch helpers
+// Small grids: a launch of few blocks (a 64^3 box's passes have 43-66 column
+// tiles of 32 columns, its z pass 64 blocks of 64 lines) leaves most CUs
+// idle, so the launcher divides the tile (columns, or lines per block of the
+// z kernels) by D in {1, 2, 4}, doubling D while the launch has fewer than
+// 512 blocks.  D > 1 is compiled for lines of up to 128 points (z: 64) and
+// only while a block stays whole waves.
+constexpr int max_div(int len, int lim, int T, int threads) {
+  int d = 1;
+  while (2 * d <= 4 && len <= lim && T % (2 * d) == 0 && threads / (2 * d) >= 64 && threads % (128 * d) == 0) d *= 2;
+  return d;
+}
+inline int tile_div(long blocks1, int maxd) {
+  int d = 1;
+  while (2 * d <= maxd && blocks1 * d < 512) d *= 2;
+  return d;
+}
+// Calls go(std::integral_constant<int, D>) for the runtime divisor d (1, 2, 4 <= MD).
+template <int MD, typename F> void with_div(int d, F go) {
+  if constexpr (MD >= 4) {
+    if (d == 4) return go(std::integral_constant<int, 4>{});
+  }
+  if constexpr (MD >= 2) {
+    if (d == 2) return go(std::integral_constant<int, 2>{});
+  }
+  go(std::integral_constant<int, 1>{});
+}
 template <typename Rt, int M, int E> void p1_go(const P1Args<Rt>& a, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
-  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
-  hipLaunchKernelGGL((k_p1<Rt, M, E>), dim3(xcd_grid(nt), a.w ? 6 : 3), dim3(Cfg::THREADS), 0, s, a);
+  constexpr int MD = max_div(M, 128, Cfg::T, Cfg::THREADS);
+  const int ny = a.w ? 6 : 3;
+  with_div<MD>(tile_div((long)ny * ((a.Q + Cfg::T - 1) / Cfg::T), MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, T = Cfg::T / D;
+    hipLaunchKernelGGL((k_p1<Rt, M, E, D>), dim3(xcd_grid((a.Q + T - 1) / T), ny), dim3(Cfg::THREADS / D), 0, s, a);
+  });
 }
 template <typename Rt, int M, int E> void p2_go(const P2Args<Rt>& a, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
-  const int nt = (a.Q + Cfg::T - 1) / Cfg::T;
-  hipLaunchKernelGGL((k_p2<Rt, M, E>), dim3(xcd_grid(nt), 2), dim3(Cfg::THREADS), 0, s, a);
+  constexpr int MD = max_div(M, 128, Cfg::T, Cfg::THREADS);
+  with_div<MD>(tile_div(2L * ((a.Q + Cfg::T - 1) / Cfg::T), MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, T = Cfg::T / D;
+    hipLaunchKernelGGL((k_p2<Rt, M, E, D>), dim3(xcd_grid((a.Q + T - 1) / T), 2), dim3(Cfg::THREADS / D), 0, s, a);
+  });
 }
 template <typename Rt, int n, int E> void p3_go(const P3Args<Rt>& a, hipStream_t s) {
   using Cfg = ZCfg<Rt, n, E>;
-  hipLaunchKernelGGL((k_p3<Rt, n, E>), dim3((a.L + Cfg::LPB - 1) / Cfg::LPB), dim3(Cfg::THREADS), 0, s, a);
+  constexpr int MD = max_div(n, 64, Cfg::LPB, Cfg::THREADS);
+  with_div<MD>(tile_div((a.L + Cfg::LPB - 1) / Cfg::LPB, MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, LPB = Cfg::LPB / D;
+    hipLaunchKernelGGL((k_p3<Rt, n, E, D>), dim3((a.L + LPB - 1) / LPB), dim3(Cfg::THREADS / D), 0, s, a);
+  });
 }
 // The LDS-assisted z pass, specialised (for n >= 64) on the dealiased bin
 // counts: the 3/2 rule holds 2n/3 input bins and needs 2n/3 + 1 outputs, the
@@ -96,12 +135,19 @@ template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, h
       return;
     }
   }
-  switch (mode) {
-    case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RHS>), g, b, 0, s, a); break;
-    case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK0>), g, b, 0, s, a); break;
-    case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK12>), g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK3>), g, b, 0, s, a); break;
-  }
+  constexpr int MD = max_div(M, 128, Cfg::T, Cfg::THREADS);
+  with_div<MD>(tile_div(nt, MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, T = Cfg::T / D;
+    const int ntd = (a.Q + T - 1) / T;
+    if (nblk) *nblk = ntd;
+    const dim3 gd(xcd_grid(ntd)), bd(Cfg::THREADS / D);
+    switch (mode) {
+      case P5_RHS: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RHS, false, D>), gd, bd, 0, s, a); break;
+      case P5_RK0: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK0, false, D>), gd, bd, 0, s, a); break;
+      case P5_RK12: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK12, false, D>), gd, bd, 0, s, a); break;
+      default: hipLaunchKernelGGL((k_p5<Rt, M, E, P5_RK3, false, D>), gd, bd, 0, s, a); break;
+    }
+  });
 }
 // Every column tile of T columns at element offsets hi * hs + lo (lo < cdiv),
 // rows rs and fields fs apart, covers whole 128-byte lines of `p`.
@@ -113,18 +159,26 @@ template <typename Rt> bool tiles_whole_lines(const void* p, int T, long cdiv, l
 template <typename Rt, int M, int E> void strided_go(int dir, int nfields, const StridedArgs<Rt>& a0, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a0.Q + Cfg::T - 1) / Cfg::T;
-  const dim3 g(xcd_grid(nt), nfields), b(Cfg::THREADS);
-  StridedArgs<Rt> a = a0;
-  a.stream_in = tiles_whole_lines<Rt>(a.in, Cfg::T, a.cdiv, a.in_hi, a.in_row, a.in_field);
-  a.stream_out = tiles_whole_lines<Rt>(a.out, Cfg::T, a.cdiv, a.out_hi, a.out_row, a.out_field);
-  if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, M, E, -1>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_strided<Rt, M, E, 1>), g, b, 0, s, a);
+  constexpr int MD = max_div(M, 128, Cfg::T, Cfg::THREADS);
+  with_div<MD>(tile_div((long)nfields * nt, MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, T = Cfg::T / D;
+    StridedArgs<Rt> a = a0;
+    a.stream_in = tiles_whole_lines<Rt>(a.in, T, a.cdiv, a.in_hi, a.in_row, a.in_field);
+    a.stream_out = tiles_whole_lines<Rt>(a.out, T, a.cdiv, a.out_hi, a.out_row, a.out_field);
+    const dim3 gd(xcd_grid((a.Q + T - 1) / T), nfields), bd(Cfg::THREADS / D);
+    if (dir < 0) hipLaunchKernelGGL((k_strided<Rt, M, E, -1, D>), gd, bd, 0, s, a);
+    else hipLaunchKernelGGL((k_strided<Rt, M, E, 1, D>), gd, bd, 0, s, a);
+  });
 }
 template <typename Rt, int n, int E> void z_go(int dir, int nfields, const ZArgs<Rt>& a, hipStream_t s) {
   using Cfg = ZCfg<Rt, n, E>;
-  const dim3 g((a.L + Cfg::LPB - 1) / Cfg::LPB, nfields), b(Cfg::THREADS);
-  if (dir < 0) hipLaunchKernelGGL((k_z_r2c<Rt, n, E>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_z_c2r<Rt, n, E>), g, b, 0, s, a);
+  constexpr int MD = max_div(n, 64, Cfg::LPB, Cfg::THREADS);
+  with_div<MD>(tile_div((long)nfields * ((a.L + Cfg::LPB - 1) / Cfg::LPB), MD), [&](auto dd) {
+    constexpr int D = decltype(dd)::value, LPB = Cfg::LPB / D;
+    const dim3 g((a.L + LPB - 1) / LPB, nfields), b(Cfg::THREADS / D);
+    if (dir < 0) hipLaunchKernelGGL((k_z_r2c<Rt, n, E, D>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_z_c2r<Rt, n, E, D>), g, b, 0, s, a);
+  });
 }
 
 // (variant 1 of a length with a single schedule instantiates the same kernel as variant 0)

@@ -28,7 +28,7 @@ import os
 import numpy as np
 
 SCHEDULE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedules.json")
-PASSES = ("x_inverse", "y_inverse_curl", "z_cross", "y_forward", "x_forward_rk")
+PASSES = ("x_inverse", "y_inverse_curl", "z_cross", "y_forward", "x_forward_rk", "transforms")
 
 
 def key_of(N, precision, dealias, mode=0):
@@ -62,7 +62,7 @@ def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, 
     cd = torch.complex64 if precision == "single" else torch.complex128
     u0 = torch.from_numpy(random_solenoidal_field(N, L, seed=seed, k0=4.0, energy=0.5)).to(cd).cuda()
     res = {"variants": {}, "ms": {}}
-    for pid, name in enumerate(PASSES):
+    for pid, name in enumerate(PASSES[:5]):
         times = []
         for var in ((0, 1, 2, 3, 4, 5) if pid == 2 else (0, 1, 2, 3) if pid == 4 else (0, 1)):
             plan.set_variant(pid, var)
@@ -80,6 +80,30 @@ def tune(N, L=(2 * np.pi,) * 3, precision="single", dealias="3/2-rule", mode=0, 
         plan.set_variant(pid, best)
         res["variants"][name] = best
         res["ms"][name] = [round(t, 5) for t in times]
+    # plain transforms (pass 5): a forward and a backward transform of the
+    # three velocity components, timed with events on the plan's stream
+    rd = torch.float32 if precision == "single" else torch.float64
+    phys = torch.empty((3,) + tuple(N), dtype=rd, device="cuda")
+    spec = u0.clone()
+    stream = torch.cuda.current_stream()
+    times = []
+    for var in (0, 1):
+        plan.set_variant(5, var)
+        plan.backward(spec, phys)
+        plan.forward(phys, spec)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            plan.backward(spec, phys)
+            plan.forward(phys, spec)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) / steps)
+    best = int(np.argmin(times))
+    plan.set_variant(5, best)
+    res["variants"]["transforms"] = best
+    res["ms"]["transforms"] = [round(t, 5) for t in times]
     return res
 
 

@@ -232,10 +232,12 @@ def test_256_cube_invariants(precision, rule):
 
 
 @pytest.mark.parametrize("N,rule", [((128, 64, 256), "2/3-rule"), ((128, 128, 128), "3/2-rule"),
-                                    ((64, 32, 256), "3/2-rule")])
+                                    ((64, 32, 256), "3/2-rule"), ((64, 64, 64), "2/3-rule")])
 def test_alternative_schedules_fp64(N, rule):
     """Every pass run with its alternative line-FFT schedule (variant 1) keeps
-    FP64 parity with the oracle, for the RHS and the plain transforms."""
+    FP64 parity with the oracle, for the RHS and the plain transforms (64^3:
+    the 4-point schedules of the 64- and 32-point lines, on the small-grid
+    tile divisors)."""
     L = (TWO_PI, 3.0, 5.0)
     nu = 0.01
     p = plan_of(N, L, "double", rule)

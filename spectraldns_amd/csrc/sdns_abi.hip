@@ -880,6 +880,12 @@ template <typename Rt> int batch_upload(sdns_batch* b) {
     h2.push_back(p2_args<Rt>(p));
     h3.push_back(p3_args<Rt>(p));
     h4.push_back(p4_args<Rt>(p));
+    {  // the batched y-forward streams W3 / W4 when its tiles cover whole lines
+      StridedArgs<Rt>& q = h4.back();
+      const int T = batch_T<Rt>(p->M[1], batch_variant(b, 3, p->M[1]));
+      q.stream_in = tiles_whole_lines<Rt>(q.in, T, q.cdiv, q.in_hi, q.in_row, q.in_field);
+      q.stream_out = tiles_whole_lines<Rt>(q.out, T, q.cdiv, q.out_hi, q.out_row, q.out_field);
+    }
     const Rt dt = (Rt)b->dt[i];
     for (int rk = 0; rk < 4; ++rk) {  // stages 1 and 2 share a but not b
       P5Args<Rt> pa = p5_args<Rt>(p, b->nu[i]);

@@ -3,6 +3,7 @@
 // these with runtime line lengths.  Every launcher returns 0, or -1 when the
 // length has no compiled kernel.
 #pragma once
+#include <cstdint>
 #include <hip/hip_runtime.h>
 
 #include "sdns_lineplan.hpp"
@@ -125,6 +126,14 @@ template <typename Rt> constexpr int batch_LPB(int n, int v = 0) {
   return batch_zE(n, v) ? batch_zthreads<Rt>() * batch_zE(n, v) / n : 0;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// Every column tile of T columns at element offsets hi * hs + lo (lo < cdiv),
+// rows rs and fields fs apart, covers whole 128-byte lines of `p`: then the
+// strided kernels stream the buffer (StridedArgs::stream_in / stream_out).
+template <typename Rt> inline bool tiles_whole_lines(const void* p, int T, long cdiv, long hs, long rs, long fs) {
+  const long esz = 2 * (long)sizeof(Rt);
+  return T > 0 && T * esz % 128 == 0 && cdiv % T == 0 && hs * esz % 128 == 0 && rs * esz % 128 == 0 &&
+         fs * esz % 128 == 0 && reinterpret_cast<uintptr_t>(p) % 128 == 0;
+}
 // LDS (complex elements) of a strided batch kernel compiled for lengths up
 // to lmax: tile + twiddle table
 template <typename Rt> constexpr int batch_strided_lds(int lmax) {

@@ -149,13 +149,6 @@ template <typename Rt, int M, int E> void p5_go(int mode, const P5Args<Rt>& a, h
     }
   });
 }
-// Every column tile of T columns at element offsets hi * hs + lo (lo < cdiv),
-// rows rs and fields fs apart, covers whole 128-byte lines of `p`.
-template <typename Rt> bool tiles_whole_lines(const void* p, int T, long cdiv, long hs, long rs, long fs) {
-  const long esz = 2 * (long)sizeof(Rt);
-  return T * esz % 128 == 0 && cdiv % T == 0 && hs * esz % 128 == 0 && rs * esz % 128 == 0 &&
-         fs * esz % 128 == 0 && reinterpret_cast<uintptr_t>(p) % 128 == 0;
-}
 template <typename Rt, int M, int E> void strided_go(int dir, int nfields, const StridedArgs<Rt>& a0, hipStream_t s) {
   using Cfg = SCfg<Rt, M, E>;
   const int nt = (a0.Q + Cfg::T - 1) / Cfg::T;

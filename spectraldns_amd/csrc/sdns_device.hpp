@@ -255,7 +255,7 @@ constexpr int strided_E(int M, int v = 0) {
 constexpr int zline_E(int n, int v = 0) {
   switch (n) {
     case 4: return 4;   case 6: return 6;   case 8: return 8;   case 12: return 12;
-    case 16: return 8;  case 24: return 12; case 32: return v ? 4 : 8; case 36: return 12;
+    case 16: return 8;  case 24: return 12; case 32: return v ? 2 : 8; case 36: return 12;
     case 48: return 12; case 64: return 8;  case 72: return 12; case 96: return 12;
     case 128: return v ? 16 : 8; case 144: return 12; case 192: return v ? 6 : 12; case 256: return v ? 8 : 16;
     case 288: return 12; case 384: return v ? 24 : 12; case 512: return v ? 8 : 16; case 768: return 12;
